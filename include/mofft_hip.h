@@ -1,0 +1,130 @@
+/*
+ * mofft_hip.h - additive MI355X extensions to the MoFFT C ABI.
+ *
+ * The reference has one transform per plan and no device memory
+ * (include/mofft.h:107-208); nothing here replaces a reference entry point.
+ * These calls add what a GPU caller needs on top of mofft.h:
+ *   - batched plans: `batch` transforms of `length` points stored back to
+ *     back (transform b starts at element b*length);
+ *   - stream-ordered execution on device pointers with no host sync;
+ *   - a caller-supplied workspace so one plan can run on several streams;
+ *   - host-only plan description (no device needed) for tooling and tests.
+ * Streams are passed as `void *` holding a hipStream_t (NULL = the null
+ * stream), so this header needs no HIP include.
+ */
+#ifndef MOFFT_HIP_H
+#define MOFFT_HIP_H
+
+#include "mofft.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MOFFT_MAX_PASSES 16
+
+/* Batched plan creation.  `batch` >= 1; batch == 1 is the mofft.h plan. */
+mofft_status mofft_plan_create_batched_f32(
+    mofft_plan_f32 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const mofft_plan_options *options, mofft_plan_diagnostics *diagnostics);
+mofft_status mofft_plan_create_batched_f64(
+    mofft_plan_f64 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const mofft_plan_options *options, mofft_plan_diagnostics *diagnostics);
+
+/* Batched plan with an explicit radix chain (outermost first). */
+mofft_status mofft_plan_create_batched_with_radices_f32(
+    mofft_plan_f32 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const int *radices, size_t radix_count);
+mofft_status mofft_plan_create_batched_with_radices_f64(
+    mofft_plan_f64 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const int *radices, size_t radix_count);
+
+mofft_status mofft_plan_get_batch_f32(const mofft_plan_f32 *plan,
+                                      size_t *batch);
+mofft_status mofft_plan_get_batch_f64(const mofft_plan_f64 *plan,
+                                      size_t *batch);
+
+/* Enqueue the transform on `stream`.  Input and output must be device
+ * pointers (MOFFT_INVALID_ARGUMENT otherwise).  Returns after enqueue; the
+ * plan's own workspace is used, so executions of one plan must be ordered on
+ * one stream. */
+mofft_status mofft_execute_async_f32(const mofft_plan_f32 *plan,
+                                     const mofft_complex_f32 *input,
+                                     mofft_complex_f32 *output, void *stream);
+mofft_status mofft_execute_async_f64(const mofft_plan_f64 *plan,
+                                     const mofft_complex_f64 *input,
+                                     mofft_complex_f64 *output, void *stream);
+
+/* Bytes of device workspace an execution needs (0 when the schedule needs
+ * none), and an execution that uses caller-owned workspace instead of the
+ * plan's, so one plan may run concurrently on several streams. */
+mofft_status mofft_plan_workspace_size_f32(const mofft_plan_f32 *plan,
+                                           size_t *bytes);
+mofft_status mofft_plan_workspace_size_f64(const mofft_plan_f64 *plan,
+                                           size_t *bytes);
+mofft_status mofft_execute_async_workspace_f32(
+    const mofft_plan_f32 *plan, const mofft_complex_f32 *input,
+    mofft_complex_f32 *output, void *workspace, size_t workspace_bytes,
+    void *stream);
+mofft_status mofft_execute_async_workspace_f64(
+    const mofft_plan_f64 *plan, const mofft_complex_f64 *input,
+    mofft_complex_f64 *output, void *workspace, size_t workspace_bytes,
+    void *stream);
+
+/* The GPU schedule of a plan: the chain grouped into HBM passes.  Pass 0 is
+ * executed first.  Each pass is one kernel launch per batch chunk that keeps
+ * `pass_length` points of every column in LDS. */
+typedef struct {
+  size_t length;
+  size_t batch;
+  int precision_bits;
+  size_t radix_count;
+  int radices[MOFFT_MAX_PLAN_DEPTH];      /* outermost first */
+  uint64_t pass_mask;                     /* bit l: HBM pass boundary
+                                             before level l */
+  int pass_count;
+  size_t pass_length[MOFFT_MAX_PASSES];   /* points per column, pass order */
+  size_t pass_stride[MOFFT_MAX_PASSES];   /* length / pass_length */
+  size_t pass_span[MOFFT_MAX_PASSES];     /* product of earlier passes */
+  int pass_kernel[MOFFT_MAX_PASSES];      /* kernel registry index */
+  size_t chunk_transforms;                /* transforms per launch */
+  size_t workspace_bytes;
+  size_t twiddle_bytes;
+} mofft_plan_description;
+
+/* Plan on the host only: no device is touched, so this runs anywhere.  The
+ * result is the schedule mofft_plan_create_batched_* would build with
+ * PRETUNED options (radices == NULL) or with the given chain and mask. */
+mofft_status mofft_plan_describe(size_t length, size_t batch,
+                                 int precision_bits, const int *radices,
+                                 size_t radix_count, uint64_t pass_mask,
+                                 mofft_plan_description *description);
+mofft_status mofft_plan_get_description_f32(const mofft_plan_f32 *plan,
+                                            mofft_plan_description *out);
+mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
+                                            mofft_plan_description *out);
+
+/* Kernel registry introspection (host only). */
+typedef struct {
+  int precision_bits;
+  int length;          /* points per column kept in LDS */
+  int rows;            /* 1: columns are whole contiguous transforms */
+  int threads_per_column;
+  int columns_per_block;
+  int stage_count;
+  int stages[8];       /* execution order: stage 0 first (innermost) */
+} mofft_kernel_info;
+int mofft_kernel_count(void);
+mofft_status mofft_kernel_get_info(int index, mofft_kernel_info *info);
+
+/* 1 when a gfx950 device is present and usable, else 0. */
+int mofft_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,368 @@
+"""MI355X-native MoFFT: Python bindings over the C ABI of libmofft.so.
+
+The product is the C library (``include/mofft.h`` drop-in plus
+``include/mofft_hip.h`` extensions) built by ``mofft_amd.build`` into
+``mofft_amd/lib/libmofft.so``.  This module binds it with ctypes so tests,
+the benchmark and Python users call exactly the entry points a C caller
+would (the reference's API, include/mofft.h:107-231).  There is no Python or
+CPU compute path: if the library is missing, importing the bindings raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmofft.so")
+
+# include/mofft.h:14-35
+SUCCESS = 0
+INVALID_ARGUMENT = 1
+UNSUPPORTED_SIZE = 2
+ALLOCATION_FAILURE = 3
+UNSUPPORTED_HARDWARE = 4
+INTERNAL_ERROR = 5
+WISDOM_NOT_FOUND = 6
+INCOMPATIBLE_RECIPE = 7
+BUFFER_TOO_SMALL = 8
+FORWARD = -1
+BACKWARD = 1
+OUT_OF_PLACE = 0
+IN_PLACE = 1
+PRETUNED = 0
+MEASURE = 1
+STRICT_WISDOM = 1
+FLAG_MEASURE = 1
+FLAG_WISDOM_ONLY = 2
+MAX_PLAN_DEPTH = 64
+MAX_PASSES = 16
+
+
+class MofftError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} ({status})")
+
+
+class PlanOptions(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("flags", ctypes.c_uint32),
+                ("candidate_limit", ctypes.c_size_t),
+                ("timing_samples", ctypes.c_uint32)]
+
+
+class PlanDiagnostics(ctypes.Structure):
+    _fields_ = [("measured", ctypes.c_int), ("used_wisdom", ctypes.c_int),
+                ("used_model_fallback", ctypes.c_int),
+                ("candidates_measured", ctypes.c_size_t),
+                ("radix_count", ctypes.c_size_t),
+                ("radices", ctypes.c_int * MAX_PLAN_DEPTH),
+                ("kernel_bank", ctypes.c_uint),
+                ("kernel_bank_count", ctypes.c_uint),
+                ("blocked_transpose_mask", ctypes.c_uint64),
+                ("direct_input", ctypes.c_int),
+                ("stage_layout", ctypes.c_int),
+                ("planning_seconds", ctypes.c_double),
+                ("selected_median_seconds", ctypes.c_double),
+                ("length", ctypes.c_size_t),
+                ("precision_bits", ctypes.c_int),
+                ("direction", ctypes.c_int),
+                ("placement", ctypes.c_int)]
+
+    @property
+    def chain(self) -> list[int]:
+        return list(self.radices[: self.radix_count])
+
+
+class CostBreakdown(ctypes.Structure):
+    _fields_ = [("kernel_cost", ctypes.c_double), ("memory_cost", ctypes.c_double),
+                ("layout_cost", ctypes.c_double), ("total_cost", ctypes.c_double),
+                ("logical_read_bytes", ctypes.c_size_t),
+                ("logical_write_bytes", ctypes.c_size_t),
+                ("transferred_read_bytes", ctypes.c_size_t),
+                ("transferred_write_bytes", ctypes.c_size_t),
+                ("peak_working_set_bytes", ctypes.c_size_t)]
+
+
+class PlanDescription(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_size_t), ("batch", ctypes.c_size_t),
+                ("precision_bits", ctypes.c_int),
+                ("radix_count", ctypes.c_size_t),
+                ("radices", ctypes.c_int * MAX_PLAN_DEPTH),
+                ("pass_mask", ctypes.c_uint64),
+                ("pass_count", ctypes.c_int),
+                ("pass_length", ctypes.c_size_t * MAX_PASSES),
+                ("pass_stride", ctypes.c_size_t * MAX_PASSES),
+                ("pass_span", ctypes.c_size_t * MAX_PASSES),
+                ("pass_kernel", ctypes.c_int * MAX_PASSES),
+                ("chunk_transforms", ctypes.c_size_t),
+                ("workspace_bytes", ctypes.c_size_t),
+                ("twiddle_bytes", ctypes.c_size_t)]
+
+    @property
+    def chain(self) -> list[int]:
+        return list(self.radices[: self.radix_count])
+
+    @property
+    def passes(self) -> list[int]:
+        return list(self.pass_length[: self.pass_count])
+
+
+class KernelInfo(ctypes.Structure):
+    _fields_ = [("precision_bits", ctypes.c_int), ("length", ctypes.c_int),
+                ("rows", ctypes.c_int), ("threads_per_column", ctypes.c_int),
+                ("columns_per_block", ctypes.c_int), ("stage_count", ctypes.c_int),
+                ("stages", ctypes.c_int * 8)]
+
+
+_lib = None
+
+# (name, restype, argtypes); P = void pointer
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_I = ctypes.c_int
+_U64 = ctypes.c_uint64
+_SIGS = []
+for _s in ("f32", "f64"):
+    _SIGS += [
+        (f"mofft_plan_create_{_s}", _I, [ctypes.POINTER(_P), _SZ, _I, _I]),
+        (f"mofft_plan_create_with_options_{_s}", _I,
+         [ctypes.POINTER(_P), _SZ, _I, _I, ctypes.POINTER(PlanOptions),
+          ctypes.POINTER(PlanDiagnostics)]),
+        (f"mofft_plan_dft_1d_{_s}", _P, [_SZ, _P, _P, _I, ctypes.c_uint32]),
+        (f"mofft_execute_plan_{_s}", _I, [_P]),
+        (f"mofft_plan_create_with_radices_{_s}", _I,
+         [ctypes.POINTER(_P), _SZ, _I, _I, ctypes.POINTER(_I), _SZ]),
+        (f"mofft_plan_set_transpose_strategy_{_s}", _I, [_P, _I]),
+        (f"mofft_plan_set_blocked_transpose_mask_{_s}", _I, [_P, _U64]),
+        (f"mofft_plan_set_direct_input_{_s}", _I, [_P, _I]),
+        (f"mofft_plan_set_stage_layout_{_s}", _I, [_P, _I]),
+        (f"mofft_plan_set_kernel_bank_{_s}", _I, [_P, ctypes.c_uint]),
+        (f"mofft_plan_get_kernel_bank_{_s}", _I, [_P, ctypes.POINTER(ctypes.c_uint)]),
+        (f"mofft_plan_export_{_s}", _I, [_P, ctypes.c_char_p, _SZ, ctypes.POINTER(_SZ)]),
+        (f"mofft_plan_import_{_s}", _I,
+         [ctypes.POINTER(_P), ctypes.c_char_p, _SZ, ctypes.POINTER(PlanDiagnostics)]),
+        (f"mofft_execute_{_s}", _I, [_P, _P, _P]),
+        (f"mofft_plan_destroy_{_s}", None, [_P]),
+        (f"mofft_plan_create_batched_{_s}", _I,
+         [ctypes.POINTER(_P), _SZ, _SZ, _I, _I, ctypes.POINTER(PlanOptions),
+          ctypes.POINTER(PlanDiagnostics)]),
+        (f"mofft_plan_create_batched_with_radices_{_s}", _I,
+         [ctypes.POINTER(_P), _SZ, _SZ, _I, _I, ctypes.POINTER(_I), _SZ]),
+        (f"mofft_plan_get_batch_{_s}", _I, [_P, ctypes.POINTER(_SZ)]),
+        (f"mofft_execute_async_{_s}", _I, [_P, _P, _P, _P]),
+        (f"mofft_plan_workspace_size_{_s}", _I, [_P, ctypes.POINTER(_SZ)]),
+        (f"mofft_execute_async_workspace_{_s}", _I, [_P, _P, _P, _P, _SZ, _P]),
+        (f"mofft_plan_get_description_{_s}", _I, [_P, ctypes.POINTER(PlanDescription)]),
+    ]
+_SIGS += [
+    ("mofft_plan_options_init", None, [ctypes.POINTER(PlanOptions)]),
+    ("mofft_build_id", ctypes.c_char_p, []),
+    ("mofft_estimated_radix_cost", ctypes.c_double, [_I, _I, _I]),
+    ("mofft_estimated_stage_cost", ctypes.c_double, [_I, _I, _I, _SZ, _SZ]),
+    ("mofft_estimated_plan_cost", ctypes.c_double,
+     [_SZ, _I, ctypes.POINTER(_I), _SZ, _U64, _I]),
+    ("mofft_estimated_plan_cost_with_layout", ctypes.c_double,
+     [_SZ, _I, ctypes.POINTER(_I), _SZ, _U64, _I, _I]),
+    ("mofft_estimated_plan_cost_breakdown_with_layout", _I,
+     [_SZ, _I, ctypes.POINTER(_I), _SZ, _U64, _I, _I, ctypes.POINTER(CostBreakdown)]),
+    ("mofft_wisdom_is_formal", _I, []),
+    ("mofft_wisdom_source", ctypes.c_char_p, []),
+    ("mofft_status_string", ctypes.c_char_p, [_I]),
+    ("mofft_version", ctypes.c_char_p, []),
+    ("mofft_plan_describe", _I,
+     [_SZ, _SZ, _I, ctypes.POINTER(_I), _SZ, _U64, ctypes.POINTER(PlanDescription)]),
+    ("mofft_kernel_count", _I, []),
+    ("mofft_kernel_get_info", _I, [_I, ctypes.POINTER(KernelInfo)]),
+    ("mofft_device_available", _I, []),
+]
+
+
+def lib() -> ctypes.CDLL:
+    """Load libmofft.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python -m mofft_amd.build`")
+    handle = ctypes.CDLL(LIB_PATH)
+    for name, res, args in _SIGS:
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return handle
+
+
+def exported_symbols() -> list[str]:
+    return [name for name, _, _ in _SIGS]
+
+
+def status_string(status: int) -> str:
+    return lib().mofft_status_string(int(status)).decode()
+
+
+def build_id() -> str:
+    return lib().mofft_build_id().decode()
+
+
+def device_available() -> bool:
+    return bool(lib().mofft_device_available())
+
+
+def _check(status: int, what: str) -> None:
+    if status != SUCCESS:
+        raise MofftError(status, what)
+
+
+def _int_array(values: Sequence[int]):
+    arr = (ctypes.c_int * max(1, len(values)))(*values)
+    return arr, len(values)
+
+
+def describe(length: int, batch: int = 1, precision_bits: int = 32,
+             radices: Optional[Sequence[int]] = None, pass_mask: int = 0) -> PlanDescription:
+    """Host-only schedule of a plan (no device needed)."""
+    d = PlanDescription()
+    if radices is None:
+        st = lib().mofft_plan_describe(length, batch, precision_bits, None, 0, 0,
+                                       ctypes.byref(d))
+    else:
+        arr, n = _int_array(radices)
+        st = lib().mofft_plan_describe(length, batch, precision_bits, arr, n,
+                                       pass_mask, ctypes.byref(d))
+    _check(st, "mofft_plan_describe")
+    return d
+
+
+def kernels() -> list[KernelInfo]:
+    out = []
+    for i in range(lib().mofft_kernel_count()):
+        k = KernelInfo()
+        _check(lib().mofft_kernel_get_info(i, ctypes.byref(k)), "kernel_get_info")
+        out.append(k)
+    return out
+
+
+def _ptr(x) -> int:
+    """Data pointer of a torch tensor (device or host) or a numpy array."""
+    if hasattr(x, "data_ptr"):
+        return int(x.data_ptr())
+    return int(x.ctypes.data)
+
+
+class Plan:
+    """A batched 1-D C2C plan (mofft_plan_create_batched_*)."""
+
+    def __init__(self, length: int, batch: int = 1, precision_bits: int = 32,
+                 direction: int = FORWARD, in_place: bool = False,
+                 radices: Optional[Sequence[int]] = None,
+                 mode: int = PRETUNED, flags: int = 0, candidate_limit: int = 0,
+                 timing_samples: int = 0, pass_mask: Optional[int] = None):
+        if precision_bits not in (32, 64):
+            raise ValueError("precision_bits must be 32 or 64")
+        self.length = int(length)
+        self.batch = int(batch)
+        self.precision_bits = precision_bits
+        self.direction = direction
+        self.in_place = bool(in_place)
+        self._s = "f32" if precision_bits == 32 else "f64"
+        self._handle = ctypes.c_void_p()
+        self.diagnostics = PlanDiagnostics()
+        placement = IN_PLACE if in_place else OUT_OF_PLACE
+        L = lib()
+        if radices is None:
+            opts = PlanOptions()
+            L.mofft_plan_options_init(ctypes.byref(opts))
+            opts.mode = mode
+            opts.flags = flags
+            opts.candidate_limit = candidate_limit
+            opts.timing_samples = timing_samples
+            st = getattr(L, f"mofft_plan_create_batched_{self._s}")(
+                ctypes.byref(self._handle), self.length, self.batch, direction,
+                placement, ctypes.byref(opts), ctypes.byref(self.diagnostics))
+        else:
+            arr, n = _int_array(radices)
+            st = getattr(L, f"mofft_plan_create_batched_with_radices_{self._s}")(
+                ctypes.byref(self._handle), self.length, self.batch, direction,
+                placement, arr, n)
+        _check(st, f"plan N={length} batch={batch} f{precision_bits}")
+        if pass_mask is not None:
+            self.set_blocked_transpose_mask(pass_mask)
+
+    @property
+    def handle(self) -> int:
+        return self._handle.value
+
+    def _fn(self, name):
+        return getattr(lib(), f"{name}_{self._s}")
+
+    def description(self) -> PlanDescription:
+        d = PlanDescription()
+        _check(self._fn("mofft_plan_get_description")(self._handle, ctypes.byref(d)),
+               "get_description")
+        return d
+
+    def workspace_size(self) -> int:
+        n = ctypes.c_size_t()
+        _check(self._fn("mofft_plan_workspace_size")(self._handle, ctypes.byref(n)),
+               "workspace_size")
+        return n.value
+
+    def set_blocked_transpose_mask(self, mask: int) -> None:
+        _check(self._fn("mofft_plan_set_blocked_transpose_mask")(self._handle, mask),
+               "set_blocked_transpose_mask")
+
+    def execute(self, x_in, x_out) -> None:
+        """Synchronous mofft_execute_* (host or device buffers)."""
+        _check(self._fn("mofft_execute")(self._handle, _ptr(x_in), _ptr(x_out)),
+               "execute")
+
+    def execute_async(self, x_in, x_out, stream: int = 0) -> None:
+        """Stream-ordered execution on device buffers."""
+        _check(self._fn("mofft_execute_async")(self._handle, _ptr(x_in), _ptr(x_out),
+                                               stream or None), "execute_async")
+
+    def execute_async_workspace(self, x_in, x_out, workspace, nbytes: int,
+                                stream: int = 0) -> None:
+        _check(self._fn("mofft_execute_async_workspace")(
+            self._handle, _ptr(x_in), _ptr(x_out),
+            _ptr(workspace) if workspace is not None else None, nbytes,
+            stream or None), "execute_async_workspace")
+
+    def export(self) -> str:
+        need = ctypes.c_size_t()
+        _check(self._fn("mofft_plan_export")(self._handle, None, 0, ctypes.byref(need)),
+               "export size")
+        buf = ctypes.create_string_buffer(need.value)
+        _check(self._fn("mofft_plan_export")(self._handle, buf, need.value,
+                                             ctypes.byref(need)), "export")
+        return buf.value.decode()
+
+    def destroy(self) -> None:
+        if self._handle and self._handle.value:
+            self._fn("mofft_plan_destroy")(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+def estimated_plan_breakdown(length: int, precision_bits: int, radices,
+                             mask: int = 0, direct_input: int = 0,
+                             section: bool = False) -> CostBreakdown:
+    arr, n = _int_array(radices)
+    b = CostBreakdown()
+    _check(lib().mofft_estimated_plan_cost_breakdown_with_layout(
+        length, precision_bits, arr, n, mask, direct_input, int(section),
+        ctypes.byref(b)), "estimated_plan_cost_breakdown")
+    return b

@@ -1,0 +1,166 @@
+"""Emit the generated HIP/C sources for gfx950.
+
+Outputs (all deterministic for a given registry):
+
+* ``mofft_codelets.h``    - ``mofft::Dft<R>::run<T>(re, im)`` straight-line
+  forward codelets for R in the reference radix set (emitter.py:283-466 is
+  the SME counterpart);
+* ``mofft_registry.c``    - the host-side registry table (plain C);
+* ``mofft_kernels_pN.hip`` - explicit instantiations of the pass kernel and
+  one launcher per registry entry, split into parts for parallel builds;
+* ``mofft_launch_table.cpp`` - the host-only launcher table indexed by registry index
+  (the counterpart of the reference's generated dispatchers,
+  emitter.py:557-620);
+* ``manifest.json``       - entry list and op counts.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+from pathlib import Path
+
+from . import codelets, registry
+from .graph import Const, live_nodes, op_count
+
+HEADER = "/* Generated by mofft_amd/compiler/mofft_compiler/emit.py - do not edit. */\n"
+
+
+def _lit(c: Const) -> str:
+    return f"kc<T>({float(c.f64).hex()}, {float(c.f32).hex()}f)"
+
+
+def emit_codelet(r: int) -> tuple[str, dict]:
+    g, y = codelets.build(r)
+    outs = [n for c in y for n in (c.re, c.im)]
+    lines = [f"template <> struct Dft<{r}> {{",
+             "  template <typename T>",
+             f"  static __device__ __forceinline__ void run(T (&re)[{r}], T (&im)[{r}]) {{"]
+    names: dict[int, str] = {}
+    for node in live_nodes(outs):
+        if node.kind == "in":
+            kind, idx = node.name[0], int(node.name[1:])
+            var = f"x{kind}{idx}"
+            arr = "re" if kind == "r" else "im"
+            lines.append(f"    const T {var} = {arr}[{idx}];")
+            names[node.index] = var
+        elif node.kind == "const":
+            names[node.index] = _lit(node.const)
+        else:
+            var = f"v{node.index}"
+            a = [names[x.index] for x in node.args]
+            if node.kind == "add":
+                expr = f"{a[0]} + {a[1]}"
+            elif node.kind == "sub":
+                expr = f"{a[0]} - {a[1]}"
+            elif node.kind == "mul":
+                expr = f"{a[0]} * {a[1]}"
+            else:
+                expr = f"-{a[0]}"
+            lines.append(f"    const T {var} = {expr};")
+            names[node.index] = var
+    for k, c in enumerate(y):
+        lines.append(f"    re[{k}] = {names[c.re.index]};")
+        lines.append(f"    im[{k}] = {names[c.im.index]};")
+    lines += ["  }", "};", ""]
+    return "\n".join(lines), op_count(outs)
+
+
+def emit_codelets_header() -> tuple[str, dict]:
+    parts = [HEADER, "#pragma once", "#include <hip/hip_runtime.h>", "",
+             "namespace mofft {", "",
+             "template <typename T>",
+             "__device__ __forceinline__ constexpr T kc(double d, float f) {",
+             "  if constexpr (sizeof(T) == 4) return f; else return d;",
+             "}", "",
+             "template <int R> struct Dft;", ""]
+    counts = {}
+    for r in codelets.RADICES:
+        if r > 16:
+            continue  # radix 32/64 run as two-stage L = 32/64 pass kernels
+        text, oc = emit_codelet(r)
+        parts.append(text)
+        counts[r] = oc
+    parts += ["}  // namespace mofft", ""]
+    return "\n".join(parts), counts
+
+
+def _targs(e: registry.Entry) -> str:
+    t = "float" if e.precision == 32 else "double"
+    stages = ", ".join(str(r) for r in e.stages)
+    return f"{t}, {e.length}, {e.tpt}, {e.cols}, {'true' if e.rows else 'false'}, {stages}"
+
+
+def emit_registry_c(es: list[registry.Entry]) -> str:
+    lines = [HEADER, '#include "../mofft_abi.h"', "",
+             "const mofft_registry_entry mofft_registry[] = {"]
+    for e in es:
+        st = ", ".join(str(r) for r in e.stages)
+        lines.append(f"  {{{e.precision}, {e.length}, {e.rows}, {e.tpt}, {e.cols}, "
+                     f"{len(e.stages)}, {{{st}}}, {e.lds_bytes}}},")
+    lines += ["};", f"const int mofft_registry_count = {len(es)};", ""]
+    return "\n".join(lines)
+
+
+def emit_kernel_part(es: list[registry.Entry], indices: list[int]) -> str:
+    lines = [HEADER, '#include "../hip/fft_pass.h"', "", "namespace mofft {"]
+    for i in indices:
+        e = es[i]
+        lines.append(f"int launch_entry_{i}(const mofft_pass_args *a, void *s) {{"
+                     f" return PassKernel<{_targs(e)}>::launch(a, (hipStream_t)s); }}")
+    lines += ["}  // namespace mofft", ""]
+    return "\n".join(lines)
+
+
+def emit_launch_table(es: list[registry.Entry]) -> str:
+    lines = [HEADER, '#include "../mofft_abi.h"', "",
+             "namespace mofft {"]
+    for i in range(len(es)):
+        lines.append(f"int launch_entry_{i}(const mofft_pass_args *, void *);")
+    lines += ["", "typedef int (*launch_fn)(const mofft_pass_args *, void *);",
+              "extern const launch_fn launch_table[] = {"]
+    for i in range(len(es)):
+        lines.append(f"  launch_entry_{i},")
+    lines += ["};", f"extern const int launch_table_size = {len(es)};",
+              "}  // namespace mofft", ""]
+    return "\n".join(lines)
+
+
+def emit(out_dir: str | Path, parts: int = 8) -> dict:
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    es = registry.entries()
+    registry.check(es)
+    files: dict[str, str] = {}
+    files["mofft_codelets.h"], counts = emit_codelets_header()
+    files["mofft_registry.c"] = emit_registry_c(es)
+    # balance parts by an estimate of compile cost (threads * stages)
+    order = sorted(range(len(es)), key=lambda i: -(es[i].length * len(es[i].stages)))
+    buckets: list[list[int]] = [[] for _ in range(parts)]
+    loads = [0] * parts
+    for i in order:
+        k = loads.index(min(loads))
+        buckets[k].append(i)
+        loads[k] += es[i].length * len(es[i].stages) + 64
+    for k, idx in enumerate(buckets):
+        files[f"mofft_kernels_p{k}.hip"] = emit_kernel_part(es, sorted(idx))
+    files["mofft_launch_table.cpp"] = emit_launch_table(es)
+    digest = hashlib.sha256()
+    for name in sorted(files):
+        digest.update(name.encode())
+        digest.update(files[name].encode())
+    manifest = {
+        "schema": 1,
+        "target": "gfx950",
+        "entries": [{"precision": e.precision, "length": e.length, "rows": e.rows,
+                     "threads_per_column": e.tpt, "columns_per_block": e.cols,
+                     "stages": list(e.stages), "lds_bytes": e.lds_bytes}
+                    for e in es],
+        "codelet_ops": {str(k): v for k, v in counts.items()},
+        "digest": digest.hexdigest(),
+    }
+    files["manifest.json"] = json.dumps(manifest, indent=1, sort_keys=True) + "\n"
+    for name, text in files.items():
+        path = out / name
+        if not path.exists() or path.read_text() != text:
+            path.write_text(text)
+    return manifest

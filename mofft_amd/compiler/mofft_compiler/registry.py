@@ -1,0 +1,192 @@
+"""The gfx950 pass-kernel registry: which LDS-resident kernels get compiled.
+
+An entry is one instantiation of ``fft_pass_kernel`` (csrc/hip/fft_pass.h):
+
+* ``length`` L: points per column kept in LDS;
+* ``stages``: the radix chain of the L-point DFT, execution order (stage 0
+  first, i.e. innermost first - the reverse of the reference's outer-to-inner
+  radix lists, include/mofft.h:138-139);
+* ``rows``: 1 when a column is a whole contiguous transform (single-pass
+  plans), 0 for strided columns of a multi-pass plan;
+* ``tpt`` threads per column and ``cols`` columns per workgroup.
+
+The reference's radix set {64, 32, 16, 15, ..., 2} (src/mofft.c:27-28) is
+covered: every radix has a strided single-radix entry (radix 32 and 64 run as
+two-stage L = 32 / 64 kernels), so any plan chain the reference accepts has a
+schedule here.
+"""
+from __future__ import annotations
+
+import itertools
+from dataclasses import dataclass
+
+from .codelets import RADICES
+
+LDS_CAP = 96 * 1024          # bytes of dynamic LDS a registry entry may use
+STAGE_RADICES = (16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2)
+
+# lengths the reference's own benchmark and tests use (tools/benchmark.cpp:28-41,
+# tests/test_fft.c:490-491, tests/test_binary_banks.c:60-62) and their
+# four-step factors
+MIXED_ROWS = (30, 45, 121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
+MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
+
+
+@dataclass(frozen=True)
+class Entry:
+    precision: int
+    length: int
+    rows: int
+    tpt: int
+    cols: int
+    stages: tuple[int, ...]
+
+    @property
+    def complex_bytes(self) -> int:
+        return 8 if self.precision == 32 else 16
+
+    @property
+    def lds_bytes(self) -> int:
+        real = self.complex_bytes // 2
+        lp = self.length + (self.length >> 4) + 1
+        row_words = self.cols * lp
+        col_words = self.length * self.cols + ((self.length * self.cols) >> 5) + 1
+        words = row_words if self.rows else max(row_words, col_words)
+        return 2 * words * real
+
+    @property
+    def threads(self) -> int:
+        return self.tpt * self.cols
+
+    @property
+    def symbol(self) -> str:
+        kind = "rows" if self.rows else "cols"
+        chain = "x".join(str(r) for r in self.stages)
+        return f"mofft_pass_f{self.precision}_{kind}_L{self.length}_{chain}_t{self.tpt}_c{self.cols}"
+
+
+def chain_for(length: int, cap: int = 16) -> tuple[int, ...]:
+    """Fewest-stage ordered factorisation of `length` into radices <= cap.
+
+    Ties prefer the most balanced chain (largest smallest radix), then
+    larger radices first."""
+    if length == 1:
+        return ()
+    best = None
+    allowed = [r for r in STAGE_RADICES if r <= cap]
+
+    def rec(n, acc):
+        nonlocal best
+        if n == 1:
+            key = (len(acc), -min(acc), tuple(-r for r in acc))
+            if best is None or key < best[0]:
+                best = (key, tuple(acc))
+            return
+        if best is not None and len(acc) >= best[0][0]:
+            return
+        for r in allowed:
+            if n % r == 0 and (not acc or r <= acc[-1]):
+                rec(n // r, acc + [r])
+
+    rec(length, [])
+    if best is None:
+        raise ValueError(f"{length} does not factor into radices <= {cap}")
+    return best[1]
+
+
+def _pow2_chain(length: int) -> tuple[int, ...]:
+    if length <= 16:
+        return (length,)
+    if length == 32:
+        return (8, 4)
+    if length == 64:
+        return (8, 8)
+    chain = []
+    n = length
+    while n % 16 == 0 and n > 16:
+        chain.append(16)
+        n //= 16
+    if n > 1:
+        chain.append(n)
+    return tuple(chain)
+
+
+def _tpt(stages: tuple[int, ...], length: int) -> int:
+    return max(1, length // max(stages))
+
+
+def _make(precision: int, length: int, rows: int,
+          stages: tuple[int, ...]) -> Entry | None:
+    tpt = _tpt(stages, length)
+    real = 4 if precision == 32 else 8
+    if rows:
+        cols = max(1, 256 // tpt)
+    else:
+        # at least 128 B of adjacent columns per access, at least 256 threads
+        cols = max(16 if precision == 32 else 8, 256 // tpt)
+        while cols * tpt > 1024 and cols > 1:
+            cols //= 2
+    e = Entry(precision, length, rows, tpt, cols, stages)
+    while e.lds_bytes > LDS_CAP and e.cols > 1:
+        e = Entry(precision, length, rows, tpt, e.cols // 2, stages)
+    if e.lds_bytes > LDS_CAP or e.threads > 1024:
+        return None
+    del real
+    return e
+
+
+def entries() -> list[Entry]:
+    out: list[Entry] = []
+    seen = set()
+
+    def add(e):
+        if e is not None and (e.precision, e.length, e.rows, e.stages) not in seen:
+            seen.add((e.precision, e.length, e.rows, e.stages))
+            out.append(e)
+
+    for precision, rows_max, cols_max in ((32, 8192, 1024), (64, 4096, 1024)):
+        # single-pass kernels over whole transforms
+        for k in range(1, rows_max.bit_length()):
+            length = 1 << k
+            add(_make(precision, length, 1, _pow2_chain(length)))
+        for r in STAGE_RADICES:
+            add(_make(precision, r, 1, (r,)))
+        for length in MIXED_ROWS:
+            add(_make(precision, length, 1, chain_for(length)))
+        # strided kernels for multi-pass schedules
+        for k in range(1, cols_max.bit_length()):
+            length = 1 << k
+            add(_make(precision, length, 0, _pow2_chain(length)))
+        for r in STAGE_RADICES:
+            add(_make(precision, r, 0, (r,)))
+        for length in MIXED_COLS:
+            add(_make(precision, length, 0, chain_for(length)))
+    return out
+
+
+def check(es: list[Entry]) -> None:
+    for e in es:
+        prod = 1
+        for r in e.stages:
+            assert r in RADICES and r <= 16, e
+            prod *= r
+        assert prod == e.length, e
+        assert e.threads <= 1024 and e.lds_bytes <= LDS_CAP, e
+    # every reference radix has a strided single-radix (or L = radix) entry
+    for precision in (32, 64):
+        for r in RADICES:
+            assert any(e.precision == precision and e.length == r and not e.rows
+                       for e in es), (precision, r)
+
+
+def describe(es: list[Entry]) -> str:
+    lines = []
+    for i, e in enumerate(es):
+        lines.append(f"{i:3d} f{e.precision} {'rows' if e.rows else 'cols'} "
+                     f"L={e.length:5d} stages={list(e.stages)} tpt={e.tpt} "
+                     f"cols={e.cols} threads={e.threads} lds={e.lds_bytes}")
+    return "\n".join(lines)
+
+
+__all__ = ["Entry", "entries", "check", "describe", "chain_for"]
+_ = itertools  # kept for callers that enumerate chains

@@ -1,0 +1,160 @@
+/*
+ * device.hip - the HIP side of the C ABI declared in mofft_abi.h.
+ *
+ * Device discovery, allocation, copies, FP64 twiddle generation and the pass
+ * launcher.  This file and the generated kernel parts are the only code that
+ * talks to HIP; mofft.c stays plain C.  It replaces the reference's
+ * hardware probe and streaming-mode entry (src/mofft.c:89-113,
+ * src/streaming_entry.S:4-38).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "../mofft_abi.h"
+
+namespace mofft {
+typedef int (*launch_fn)(const mofft_pass_args *, void *);
+extern const launch_fn launch_table[];
+extern const int launch_table_size;
+
+/* exp(-2*pi*i*m/M) in FP64: exact integer octant reduction of 8m/8M, then
+ * sincospi of an argument in [0, 1/4]. */
+__device__ __forceinline__ void omega_f64(uint64_t m, uint64_t M, double &c,
+                                          double &s) {
+  m %= M;
+  const uint64_t Q = 8 * M;
+  uint64_t p = 8 * m;
+  bool neg_s = false, neg_c = false, swap = false;
+  if (p > Q / 2) { p = Q - p; neg_s = true; }
+  if (p > Q / 4) { p = Q / 2 - p; neg_c = true; }
+  if (p > Q / 8) { p = Q / 4 - p; swap = true; }
+  double sv, cv;
+  sincospi(2.0 * ((double)p / (double)Q), &sv, &cv);
+  if (swap) { const double t = cv; cv = sv; sv = t; }
+  if (neg_c) cv = -cv;
+  if (neg_s) sv = -sv;
+  c = cv;
+  s = -sv; /* forward sign */
+}
+
+template <typename T>
+__global__ void fill_powers_kernel(T *out, int64_t count, uint64_t M,
+                                   uint64_t step) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint64_t m = (uint64_t)(((unsigned __int128)i * step) % M);
+  double c, s;
+  omega_f64(m, M, c, s);
+  out[2 * i] = (T)c;
+  out[2 * i + 1] = (T)s;
+}
+
+template <typename T>
+__global__ void fill_stage_kernel(T *out, int64_t ns, int r) {
+  const int64_t count = ns * (r - 1);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t a = i / (r - 1);
+  const int64_t k = i - a * (r - 1) + 1;
+  double c, s;
+  omega_f64((uint64_t)(a * k), (uint64_t)(ns * r), c, s);
+  out[2 * i] = (T)c;
+  out[2 * i + 1] = (T)s;
+}
+}  // namespace mofft
+
+extern "C" {
+
+int mofft_hip_device_available(void) {
+  static int cached = -1;
+  if (cached >= 0) return cached;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    cached = 0;
+    return 0;
+  }
+  int device = 0;
+  if (hipGetDevice(&device) != hipSuccess) { cached = 0; return 0; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { cached = 0; return 0; }
+  cached = strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
+  return cached;
+}
+
+int mofft_hip_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return (int)hipErrorInvalidValue;
+  *ptr = nullptr;
+  if (bytes == 0) return 0;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) { *ptr = nullptr; (void)hipGetLastError(); }
+  return (int)e;
+}
+
+int mofft_hip_free(void *ptr) {
+  if (!ptr) return 0;
+  return (int)hipFree(ptr);
+}
+
+int mofft_hip_is_device_pointer(const void *ptr) {
+  hipPointerAttribute_t attr;
+  memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+int mofft_hip_copy(void *dst, const void *src, size_t bytes) {
+  if (bytes == 0) return 0;
+  return (int)hipMemcpy(dst, src, bytes, hipMemcpyDefault);
+}
+
+int mofft_hip_copy_async(void *dst, const void *src, size_t bytes,
+                         void *stream) {
+  if (bytes == 0) return 0;
+  return (int)hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault,
+                             (hipStream_t)stream);
+}
+
+int mofft_hip_synchronize(void *stream) {
+  return (int)hipStreamSynchronize((hipStream_t)stream);
+}
+
+int mofft_hip_fill_powers(void *out, int precision_bits, int64_t count,
+                          int64_t M, int64_t step, void *stream) {
+  if (count <= 0) return 0;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
+  if (precision_bits == 32)
+    hipLaunchKernelGGL(mofft::fill_powers_kernel<float>, dim3(blocks),
+                       dim3(256), 0, (hipStream_t)stream, (float *)out, count,
+                       (uint64_t)M, (uint64_t)step);
+  else
+    hipLaunchKernelGGL(mofft::fill_powers_kernel<double>, dim3(blocks),
+                       dim3(256), 0, (hipStream_t)stream, (double *)out, count,
+                       (uint64_t)M, (uint64_t)step);
+  return (int)hipGetLastError();
+}
+
+int mofft_hip_fill_stage(void *out, int precision_bits, int64_t ns, int r,
+                         void *stream) {
+  const int64_t count = ns * (r - 1);
+  if (count <= 0) return 0;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
+  if (precision_bits == 32)
+    hipLaunchKernelGGL(mofft::fill_stage_kernel<float>, dim3(blocks),
+                       dim3(256), 0, (hipStream_t)stream, (float *)out, ns, r);
+  else
+    hipLaunchKernelGGL(mofft::fill_stage_kernel<double>, dim3(blocks),
+                       dim3(256), 0, (hipStream_t)stream, (double *)out, ns, r);
+  return (int)hipGetLastError();
+}
+
+int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
+                          void *stream) {
+  if (kernel < 0 || kernel >= mofft::launch_table_size || !args)
+    return (int)hipErrorInvalidValue;
+  return mofft::launch_table[kernel](args, stream);
+}
+
+}  // extern "C"

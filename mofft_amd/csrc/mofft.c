@@ -1,0 +1,1784 @@
+/*
+ * mofft.c - host runtime of the MI355X MoFFT library (plain C).
+ *
+ * Implements every entry point of include/mofft.h (the reference's drop-in
+ * ABI, include/mofft.h:107-231) and the additive ones of mofft_hip.h.  All
+ * device work goes through the thin C ABI of mofft_abi.h.
+ *
+ * The reference executes a plan as a sequence of radix stages with a
+ * materialising transpose between stages (src/mofft.c:1932-2160).  Here the
+ * same radix chain is grouped into HBM passes: each pass is one launch of an
+ * LDS-resident kernel that runs several Stockham radix stages on-chip, so an
+ * N <= 8192 (FP32) transform costs one read and one write of HBM and larger
+ * ones a few (a "four-step" scheme generalised to any number of passes).
+ *
+ *   chain (outer -> inner)      e.g. {4, 16, 16} for N = 1024
+ *   execution order             innermost radix first (src/mofft.c:1936)
+ *   pass boundary mask          bit l: data round-trips HBM before level l
+ *                               (the GPU meaning of the reference's
+ *                               blocked-transpose mask, include/mofft.h:152)
+ */
+#include "mofft_abi.h"
+#include "../../include/mofft.h"
+#include "../../include/mofft_hip.h"
+#include "mofft_wisdom.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#ifndef MOFFT_BINARY_ID
+#define MOFFT_BINARY_ID "0000000000000000000000000000000000000000000000000000000000000000"
+#endif
+
+#define MOFFT_DEFAULT_SEARCH_CANDIDATES 16
+#define MOFFT_DEFAULT_TIMING_SAMPLES 5
+#define MOFFT_MAX_SEARCH_CANDIDATES 64
+#define MOFFT_MAX_TIMING_SAMPLES 31
+#define MOFFT_RECIPE_MAX_BYTES 1024
+/* Device family token of recipes: the gfx target number in hex digits. */
+#define MOFFT_GPU_FAMILY 0x00000950u
+/* Multi-pass schedules stream the batch in chunks of at most this many bytes
+ * so the pass-to-pass intermediate can stay in the 256 MB Infinity Cache. */
+#define MOFFT_CHUNK_BYTES ((size_t)64 << 20)
+
+/* src/mofft.c:27-28 - the radix set the API accepts. */
+static const int k_radices[] = {64, 32, 16, 15, 14, 13, 12, 11, 10,
+                                9, 8, 7, 6, 5, 4, 3, 2};
+
+typedef struct {
+  int kernel;            /* registry index */
+  size_t length;         /* points per column in LDS */
+  size_t stride;         /* n / length */
+  size_t span;           /* product of earlier pass lengths (ns) */
+  size_t stage_offset;   /* element offset of this pass's sub-stage table */
+  size_t stage_count;
+} mofft_pass;
+
+typedef struct {
+  size_t length;
+  size_t batch;
+  int precision_bits;
+  int depth;
+  int factors[MOFFT_MAX_PLAN_DEPTH];   /* outer -> inner */
+  uint64_t pass_mask;
+  int pass_count;
+  mofft_pass passes[MOFFT_MAX_PLAN_DEPTH];
+  size_t chunk;
+  size_t workspace_bytes;
+  uint32_t split_shift;
+  size_t tw_hi_offset, tw_hi_count;
+  size_t tw_lo_offset, tw_lo_count;
+  size_t twiddle_count;                /* complex elements */
+} mofft_schedule;
+
+typedef struct mofft_plan_common {
+  mofft_schedule s;
+  int direction;
+  mofft_placement placement;
+  int direct_input;
+  mofft_stage_layout stage_layout;
+  unsigned int kernel_bank;
+  void *twiddles;
+  void *workspace;
+  void *staging_in;
+  void *staging_out;
+  const void *bound_input;
+  void *bound_output;
+  int has_bound_arrays;
+} mofft_plan_common;
+
+struct mofft_plan_f32 { mofft_plan_common c; };
+struct mofft_plan_f64 { mofft_plan_common c; };
+
+/* ------------------------------------------------------------------ */
+/* small helpers                                                        */
+/* ------------------------------------------------------------------ */
+
+static size_t sat_mul(size_t a, size_t b) {
+  return a != 0 && b > SIZE_MAX / a ? SIZE_MAX : a * b;
+}
+
+static size_t sat_add(size_t a, size_t b) {
+  return b > SIZE_MAX - a ? SIZE_MAX : a + b;
+}
+
+static double monotonic_seconds(void) {
+  struct timespec now;
+  if (clock_gettime(CLOCK_MONOTONIC, &now) != 0) return -1.0;
+  return (double)now.tv_sec + (double)now.tv_nsec * 1e-9;
+}
+
+static int is_supported_radix(int radix) {
+  for (size_t i = 0; i < sizeof(k_radices) / sizeof(k_radices[0]); ++i)
+    if (k_radices[i] == radix) return 1;
+  return 0;
+}
+
+static size_t complex_bytes(int precision_bits) {
+  return precision_bits == 32 ? 8 : 16;
+}
+
+static unsigned int kernel_bank_count(void) { return 1; }
+
+int mofft_device_available(void) { return mofft_hip_device_available(); }
+
+/* ------------------------------------------------------------------ */
+/* kernel registry lookups                                              */
+/* ------------------------------------------------------------------ */
+
+/* Registry entry whose sub-stage chain equals stages[0..count) (execution
+ * order), or, for a single radix, any entry of that length. */
+static int find_kernel(int precision_bits, int rows, const int *stages,
+                       int count) {
+  for (int i = 0; i < mofft_registry_count; ++i) {
+    const mofft_registry_entry *e = &mofft_registry[i];
+    if (e->precision_bits != precision_bits || e->rows != rows) continue;
+    if (e->stage_count == count &&
+        memcmp(e->stages, stages, (size_t)count * sizeof(int)) == 0)
+      return i;
+  }
+  if (count == 1) {
+    for (int i = 0; i < mofft_registry_count; ++i) {
+      const mofft_registry_entry *e = &mofft_registry[i];
+      if (e->precision_bits == precision_bits && e->rows == rows &&
+          e->length == stages[0])
+        return i;
+    }
+  }
+  return -1;
+}
+
+static int find_kernel_by_length(int precision_bits, int rows, size_t length) {
+  for (int i = 0; i < mofft_registry_count; ++i) {
+    const mofft_registry_entry *e = &mofft_registry[i];
+    if (e->precision_bits == precision_bits && e->rows == rows &&
+        (size_t)e->length == length)
+      return i;
+  }
+  return -1;
+}
+
+int mofft_kernel_count(void) { return mofft_registry_count; }
+
+mofft_status mofft_kernel_get_info(int index, mofft_kernel_info *info) {
+  if (!info || index < 0 || index >= mofft_registry_count)
+    return MOFFT_INVALID_ARGUMENT;
+  const mofft_registry_entry *e = &mofft_registry[index];
+  memset(info, 0, sizeof(*info));
+  info->precision_bits = e->precision_bits;
+  info->length = e->length;
+  info->rows = e->rows;
+  info->threads_per_column = e->threads_per_column;
+  info->columns_per_block = e->columns_per_block;
+  info->stage_count = e->stage_count;
+  for (int i = 0; i < e->stage_count && i < 8; ++i) info->stages[i] = e->stages[i];
+  return MOFFT_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* schedule: chain + pass mask -> passes                                */
+/* ------------------------------------------------------------------ */
+
+static mofft_status validate_chain(size_t length, const int *radices,
+                                   size_t count) {
+  if (!radices || count == 0 || count > MOFFT_MAX_PLAN_DEPTH || length < 2)
+    return MOFFT_INVALID_ARGUMENT;
+  size_t remaining = length;
+  for (size_t i = 0; i < count; ++i) {
+    if (!is_supported_radix(radices[i]) || remaining % (size_t)radices[i])
+      return MOFFT_INVALID_ARGUMENT;
+    remaining /= (size_t)radices[i];
+  }
+  return remaining == 1 ? MOFFT_SUCCESS : MOFFT_INVALID_ARGUMENT;
+}
+
+/* Group the chain into passes.  Level boundaries named in `mask` always
+ * split; elsewhere each pass takes the longest run of stages that a compiled
+ * kernel implements (greedy from the innermost stage). */
+static mofft_status build_schedule(size_t length, size_t batch,
+                                   int precision_bits, const int *radices,
+                                   size_t depth, uint64_t mask,
+                                   mofft_schedule *s) {
+  mofft_status st = validate_chain(length, radices, depth);
+  if (st != MOFFT_SUCCESS) return st;
+  if (batch == 0 || (precision_bits != 32 && precision_bits != 64))
+    return MOFFT_INVALID_ARGUMENT;
+  if (depth < 64 && depth > 0 && (mask >> (depth - 1)) != 0)
+    return MOFFT_INVALID_ARGUMENT;
+  memset(s, 0, sizeof(*s));
+  s->length = length;
+  s->batch = batch;
+  s->precision_bits = precision_bits;
+  s->depth = (int)depth;
+  memcpy(s->factors, radices, depth * sizeof(int));
+  int exec[MOFFT_MAX_PLAN_DEPTH];
+  for (size_t i = 0; i < depth; ++i) exec[i] = radices[depth - 1 - i];
+  size_t start = 0, span = 1;
+  uint64_t used_mask = 0;
+  while (start < depth) {
+    /* forced boundary: execution index i = depth-1-l starts level l */
+    size_t limit = depth - start;
+    for (size_t len = 1; len < depth - start; ++len) {
+      const size_t level = depth - 1 - (start + len);
+      if ((mask >> level) & 1u) { limit = len; break; }
+    }
+    int kernel = -1;
+    size_t take = 0;
+    for (size_t len = limit; len >= 1; --len) {
+      const int rows = (start == 0 && len == depth) ? 1 : 0;
+      if (len > 8) continue;
+      kernel = find_kernel(precision_bits, rows, exec + start, (int)len);
+      if (kernel >= 0) { take = len; break; }
+    }
+    if (kernel < 0) return MOFFT_UNSUPPORTED_SIZE;
+    mofft_pass *p = &s->passes[s->pass_count++];
+    size_t pl = 1;
+    for (size_t i = 0; i < take; ++i) pl *= (size_t)exec[start + i];
+    p->kernel = kernel;
+    p->length = pl;
+    p->stride = length / pl;
+    p->span = span;
+    span *= pl;
+    start += take;
+    if (start < depth) used_mask |= (uint64_t)1 << (depth - 1 - start);
+  }
+  s->pass_mask = used_mask;
+  /* twiddle layout: pass twiddles (two-level table of w_N), then the
+   * sub-stage tables of every pass */
+  size_t count = 0;
+  if (s->pass_count > 1) {
+    uint32_t bits = 0;
+    while (((size_t)1 << bits) < length) ++bits;
+    s->split_shift = (bits + 1) / 2;
+    s->tw_lo_count = (size_t)1 << s->split_shift;
+    if (s->tw_lo_count > length) s->tw_lo_count = length;
+    s->tw_hi_count = (length + ((size_t)1 << s->split_shift) - 1) >> s->split_shift;
+    s->tw_hi_offset = 0;
+    s->tw_lo_offset = s->tw_hi_count;
+    count = s->tw_hi_count + s->tw_lo_count;
+  }
+  for (int i = 0; i < s->pass_count; ++i) {
+    mofft_pass *p = &s->passes[i];
+    const mofft_registry_entry *e = &mofft_registry[p->kernel];
+    p->stage_offset = count;
+    size_t ns = 1, n_entries = 0;
+    for (int k = 0; k < e->stage_count; ++k) {
+      if (ns > 1) n_entries += ns * (size_t)(e->stages[k] - 1);
+      ns *= (size_t)e->stages[k];
+    }
+    p->stage_count = n_entries;
+    count += n_entries;
+  }
+  s->twiddle_count = count;
+  const size_t cb = complex_bytes(precision_bits);
+  const size_t transform_bytes = sat_mul(length, cb);
+  size_t chunk = transform_bytes >= MOFFT_CHUNK_BYTES
+                     ? 1 : MOFFT_CHUNK_BYTES / transform_bytes;
+  if (chunk > batch) chunk = batch;
+  if (chunk == 0) chunk = 1;
+  s->chunk = s->pass_count > 1 ? chunk : batch;
+  s->workspace_bytes = s->pass_count > 1 ? sat_mul(s->chunk, transform_bytes) : 0;
+  return MOFFT_SUCCESS;
+}
+
+/* Workspace is needed by in-place multi-pass plans and by out-of-place
+ * plans with three or more passes. */
+static size_t schedule_workspace(const mofft_schedule *s,
+                                 mofft_placement placement) {
+  if (s->pass_count <= 1) return 0;
+  if (placement == MOFFT_OUT_OF_PLACE && s->pass_count == 2) return 0;
+  return s->workspace_bytes;
+}
+
+/* ------------------------------------------------------------------ */
+/* analytical planner (PRETUNED without wisdom)                          */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int count;
+  size_t lengths[MOFFT_MAX_PLAN_DEPTH];
+  int kernels[MOFFT_MAX_PLAN_DEPTH];
+} pass_split;
+
+/* better = fewer passes, then larger smallest pass, then larger first pass */
+static int split_better(const pass_split *a, const pass_split *b) {
+  if (b->count == 0) return 1;
+  if (a->count != b->count) return a->count < b->count;
+  size_t amin = SIZE_MAX, bmin = SIZE_MAX;
+  for (int i = 0; i < a->count; ++i) if (a->lengths[i] < amin) amin = a->lengths[i];
+  for (int i = 0; i < b->count; ++i) if (b->lengths[i] < bmin) bmin = b->lengths[i];
+  if (amin != bmin) return amin > bmin;
+  for (int i = 0; i < a->count; ++i)
+    if (a->lengths[i] != b->lengths[i]) return a->lengths[i] > b->lengths[i];
+  return 0;
+}
+
+static void search_splits(int precision_bits, size_t remaining,
+                          pass_split *cur, pass_split *best, int max_passes) {
+  if (remaining == 1) {
+    if (cur->count > 0 && split_better(cur, best)) *best = *cur;
+    return;
+  }
+  if (cur->count >= max_passes) return;
+  if (best->count != 0 && cur->count + 1 > best->count) return;
+  for (int i = 0; i < mofft_registry_count; ++i) {
+    const mofft_registry_entry *e = &mofft_registry[i];
+    if (e->precision_bits != precision_bits || e->rows) continue;
+    const size_t l = (size_t)e->length;
+    if (l < 2 || remaining % l) continue;
+    int dup = 0;
+    for (int j = 0; j < i; ++j) {
+      const mofft_registry_entry *f = &mofft_registry[j];
+      if (f->precision_bits == precision_bits && !f->rows &&
+          f->length == e->length) { dup = 1; break; }
+    }
+    if (dup) continue;
+    cur->lengths[cur->count] = l;
+    cur->kernels[cur->count] = i;
+    cur->count++;
+    search_splits(precision_bits, remaining / l, cur, best, max_passes);
+    cur->count--;
+  }
+}
+
+/* Model plan: a single-pass kernel when one covers the length, otherwise the
+ * pass decomposition with the fewest HBM round trips. */
+static mofft_status model_chain(size_t length, int precision_bits,
+                                int *radices, size_t *depth, uint64_t *mask) {
+  if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
+  const int single = find_kernel_by_length(precision_bits, 1, length);
+  if (single >= 0) {
+    const mofft_registry_entry *e = &mofft_registry[single];
+    for (int i = 0; i < e->stage_count; ++i)
+      radices[i] = e->stages[e->stage_count - 1 - i];
+    *depth = (size_t)e->stage_count;
+    *mask = 0;
+    return MOFFT_SUCCESS;
+  }
+  pass_split cur, best;
+  memset(&cur, 0, sizeof(cur));
+  memset(&best, 0, sizeof(best));
+  for (int max_passes = 2; max_passes <= 12 && best.count == 0; max_passes += 2)
+    search_splits(precision_bits, length, &cur, &best, max_passes);
+  if (best.count == 0) return MOFFT_UNSUPPORTED_SIZE;
+  /* concatenate: the last pass is outermost */
+  size_t d = 0;
+  uint64_t m = 0;
+  for (int p = best.count - 1; p >= 0; --p) {
+    const mofft_registry_entry *e = &mofft_registry[best.kernels[p]];
+    if (d + (size_t)e->stage_count > MOFFT_MAX_PLAN_DEPTH)
+      return MOFFT_UNSUPPORTED_SIZE;
+    for (int i = e->stage_count - 1; i >= 0; --i) radices[d++] = e->stages[i];
+    if (p > 0) m |= (uint64_t)1 << (d - 1);
+  }
+  *depth = d;
+  *mask = m;
+  return MOFFT_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* option handling (src/mofft.c:739-762, 1384-1396)                      */
+/* ------------------------------------------------------------------ */
+
+static mofft_status validate_options(const mofft_plan_options *given,
+                                     mofft_plan_options *resolved) {
+  if (given) *resolved = *given;
+  else mofft_plan_options_init(resolved);
+  if ((resolved->mode != MOFFT_PLANNER_PRETUNED &&
+       resolved->mode != MOFFT_PLANNER_MEASURE) ||
+      (resolved->flags & ~((uint32_t)MOFFT_PLAN_STRICT_WISDOM)) != 0 ||
+      (resolved->mode == MOFFT_PLANNER_MEASURE && resolved->flags != 0))
+    return MOFFT_INVALID_ARGUMENT;
+  if (resolved->candidate_limit == 0)
+    resolved->candidate_limit = MOFFT_DEFAULT_SEARCH_CANDIDATES;
+  if (resolved->timing_samples == 0)
+    resolved->timing_samples = MOFFT_DEFAULT_TIMING_SAMPLES;
+  if (resolved->candidate_limit > MOFFT_MAX_SEARCH_CANDIDATES ||
+      resolved->timing_samples > MOFFT_MAX_TIMING_SAMPLES)
+    return MOFFT_INVALID_ARGUMENT;
+  return MOFFT_SUCCESS;
+}
+
+void mofft_plan_options_init(mofft_plan_options *options) {
+  if (!options) return;
+  options->mode = MOFFT_PLANNER_PRETUNED;
+  options->flags = 0;
+  options->candidate_limit = 0;
+  options->timing_samples = 0;
+}
+
+static int options_from_bound_flags(uint32_t flags,
+                                    mofft_plan_options *options) {
+  if ((flags & ~((uint32_t)MOFFT_FLAG_MEASURE |
+                 (uint32_t)MOFFT_FLAG_WISDOM_ONLY)) != 0 ||
+      ((flags & MOFFT_FLAG_MEASURE) && (flags & MOFFT_FLAG_WISDOM_ONLY)))
+    return 0;
+  mofft_plan_options_init(options);
+  options->mode = (flags & MOFFT_FLAG_MEASURE) ? MOFFT_PLANNER_MEASURE
+                                               : MOFFT_PLANNER_PRETUNED;
+  if (flags & MOFFT_FLAG_WISDOM_ONLY)
+    options->flags |= MOFFT_PLAN_STRICT_WISDOM;
+  return 1;
+}
+
+/* Wisdom: exact (length, precision) entry.  1 hit, 0 miss, -1 stale. */
+static int wisdom_lookup(size_t length, int precision_bits, int *radices,
+                         size_t *depth, uint64_t *mask, int *direct,
+                         int *layout, unsigned int *bank) {
+  uint64_t m = 0;
+  int di = 0, sl = 0;
+  unsigned int b = 0;
+  const size_t count = mofft_generated_wisdom_lookup(
+      length, precision_bits, radices, MOFFT_MAX_PLAN_DEPTH, &m, &di, &sl, &b);
+  if (count == 0) return 0;
+  if (b >= kernel_bank_count()) return -1;
+  if (validate_chain(length, radices, count) != MOFFT_SUCCESS) return -1;
+  *depth = count;
+  *mask = m;
+  if (direct) *direct = di;
+  if (layout) *layout = sl;
+  if (bank) *bank = b;
+  return 1;
+}
+
+int mofft_wisdom_is_formal(void) { return mofft_generated_wisdom_is_formal(); }
+const char *mofft_wisdom_source(void) { return mofft_generated_wisdom_source(); }
+
+/* ------------------------------------------------------------------ */
+/* plan lifetime                                                        */
+/* ------------------------------------------------------------------ */
+
+static void plan_release(mofft_plan_common *p) {
+  if (!p) return;
+  mofft_hip_free(p->twiddles);
+  mofft_hip_free(p->workspace);
+  mofft_hip_free(p->staging_in);
+  mofft_hip_free(p->staging_out);
+  free(p);
+}
+
+static mofft_status fill_twiddles(mofft_plan_common *p) {
+  const mofft_schedule *s = &p->s;
+  const size_t cb = complex_bytes(s->precision_bits);
+  if (s->twiddle_count == 0) return MOFFT_SUCCESS;
+  if (mofft_hip_malloc(&p->twiddles, sat_mul(s->twiddle_count, cb)) != 0)
+    return MOFFT_ALLOCATION_FAILURE;
+  char *base = (char *)p->twiddles;
+  int rc = 0;
+  if (s->pass_count > 1) {
+    rc |= mofft_hip_fill_powers(base + s->tw_hi_offset * cb, s->precision_bits,
+                                (int64_t)s->tw_hi_count, (int64_t)s->length,
+                                (int64_t)1 << s->split_shift, NULL);
+    rc |= mofft_hip_fill_powers(base + s->tw_lo_offset * cb, s->precision_bits,
+                                (int64_t)s->tw_lo_count, (int64_t)s->length,
+                                1, NULL);
+  }
+  for (int i = 0; i < s->pass_count && rc == 0; ++i) {
+    const mofft_pass *ps = &s->passes[i];
+    const mofft_registry_entry *e = &mofft_registry[ps->kernel];
+    size_t ns = 1, off = ps->stage_offset;
+    for (int k = 0; k < e->stage_count; ++k) {
+      const int r = e->stages[k];
+      if (ns > 1) {
+        rc |= mofft_hip_fill_stage(base + off * cb, s->precision_bits,
+                                   (int64_t)ns, r, NULL);
+        off += ns * (size_t)(r - 1);
+      }
+      ns *= (size_t)r;
+    }
+  }
+  if (rc == 0) rc = mofft_hip_synchronize(NULL);
+  return rc == 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
+}
+
+/* Build a plan from an explicit chain + mask (device resources included). */
+static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
+                                    size_t batch, int precision_bits,
+                                    int direction, mofft_placement placement,
+                                    const int *radices, size_t depth,
+                                    uint64_t mask) {
+  *out = NULL;
+  if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
+  mofft_plan_common *p = (mofft_plan_common *)calloc(1, sizeof(*p));
+  if (!p) return MOFFT_ALLOCATION_FAILURE;
+  mofft_status st = build_schedule(length, batch, precision_bits, radices,
+                                   depth, mask, &p->s);
+  if (st != MOFFT_SUCCESS) { free(p); return st; }
+  p->direction = direction;
+  p->placement = placement;
+  p->direct_input = 0;
+  p->stage_layout = MOFFT_STAGE_LAYOUT_TRANSPOSE;
+  p->kernel_bank = 0;
+  st = fill_twiddles(p);
+  if (st == MOFFT_SUCCESS) {
+    const size_t ws = schedule_workspace(&p->s, placement);
+    if (ws && mofft_hip_malloc(&p->workspace, ws) != 0)
+      st = MOFFT_ALLOCATION_FAILURE;
+  }
+  if (st != MOFFT_SUCCESS) { plan_release(p); return st; }
+  *out = p;
+  return MOFFT_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* execution                                                            */
+/* ------------------------------------------------------------------ */
+
+static int launch_schedule(const mofft_plan_common *p, const void *in,
+                           void *out, void *work, void *stream) {
+  const mofft_schedule *s = &p->s;
+  const size_t cb = complex_bytes(s->precision_bits);
+  const char *tw = (const char *)p->twiddles;
+  const int backward = p->direction == MOFFT_BACKWARD;
+  const int P = s->pass_count;
+  const int in_place = in == out;
+  for (size_t c0 = 0; c0 < s->batch; c0 += s->chunk) {
+    const size_t cnt = s->batch - c0 < s->chunk ? s->batch - c0 : s->chunk;
+    const char *src_chunk = (const char *)in + c0 * s->length * cb;
+    char *dst_chunk = (char *)out + c0 * s->length * cb;
+    const void *src = src_chunk;
+    for (int i = 0; i < P; ++i) {
+      void *dst;
+      if (i == P - 1) {
+        dst = dst_chunk;
+      } else if (in_place) {
+        dst = (i % 2 == 0) ? work : dst_chunk;
+      } else {
+        dst = ((P - 2 - i) % 2 == 0) ? (void *)dst_chunk : work;
+      }
+      const mofft_pass *ps = &s->passes[i];
+      mofft_pass_args a;
+      memset(&a, 0, sizeof(a));
+      a.in = src;
+      a.out = dst;
+      a.stage_tw = tw ? tw + ps->stage_offset * cb : NULL;
+      a.tw_hi = tw ? tw + s->tw_hi_offset * cb : NULL;
+      a.tw_lo = tw ? tw + s->tw_lo_offset * cb : NULL;
+      a.n = (int64_t)s->length;
+      a.stride = (int64_t)ps->stride;
+      a.ns = (int64_t)ps->span;
+      a.tw_scale = (int64_t)(s->length / (ps->span * ps->length));
+      a.columns = (int64_t)(cnt * ps->stride);
+      a.split_shift = s->split_shift;
+      a.flags = 0;
+      if (backward && i == 0) a.flags |= MOFFT_PASS_SWAP_IN;
+      if (backward && i == P - 1) a.flags |= MOFFT_PASS_SWAP_OUT;
+      if (ps->span > 1) a.flags |= MOFFT_PASS_TWIDDLE;
+      const int rc = mofft_hip_launch_pass(ps->kernel, &a, stream);
+      if (rc != 0) return rc;
+      src = dst;
+    }
+  }
+  return 0;
+}
+
+static mofft_status execute_device(const mofft_plan_common *p, const void *in,
+                                   void *out, void *work, size_t work_bytes,
+                                   void *stream) {
+  const size_t need = schedule_workspace(&p->s, p->placement);
+  if (need && (!work || work_bytes < need)) return MOFFT_INVALID_ARGUMENT;
+  return launch_schedule(p, in, out, work, stream) == 0 ? MOFFT_SUCCESS
+                                                        : MOFFT_INTERNAL_ERROR;
+}
+
+/* Synchronous execute of mofft.h: host buffers are staged through device
+ * memory the plan keeps for later calls. */
+static mofft_status execute_sync(const mofft_plan_common *cp, const void *in,
+                                 void *out) {
+  if (!cp || !in || !out) return MOFFT_INVALID_ARGUMENT;
+  if ((cp->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
+  mofft_plan_common *p = (mofft_plan_common *)cp;
+  const size_t bytes = sat_mul(sat_mul(p->s.length, p->s.batch),
+                               complex_bytes(p->s.precision_bits));
+  const int din = mofft_hip_is_device_pointer(in);
+  const int dout = mofft_hip_is_device_pointer(out);
+  const void *dev_in = in;
+  void *dev_out = out;
+  if (!din) {
+    if (!p->staging_in && mofft_hip_malloc(&p->staging_in, bytes) != 0)
+      return MOFFT_ALLOCATION_FAILURE;
+    if (mofft_hip_copy(p->staging_in, in, bytes) != 0)
+      return MOFFT_INTERNAL_ERROR;
+    dev_in = p->staging_in;
+  }
+  if (!dout) {
+    if (in == out) {
+      dev_out = (void *)dev_in;
+    } else {
+      if (!p->staging_out && mofft_hip_malloc(&p->staging_out, bytes) != 0)
+        return MOFFT_ALLOCATION_FAILURE;
+      dev_out = p->staging_out;
+    }
+  }
+  mofft_status st = execute_device(p, dev_in, dev_out, p->workspace,
+                                   schedule_workspace(&p->s, p->placement), NULL);
+  if (st != MOFFT_SUCCESS) return st;
+  if (!dout && mofft_hip_copy(out, dev_out, bytes) != 0)
+    return MOFFT_INTERNAL_ERROR;
+  return mofft_hip_synchronize(NULL) == 0 ? MOFFT_SUCCESS
+                                          : MOFFT_INTERNAL_ERROR;
+}
+
+/* ------------------------------------------------------------------ */
+/* diagnostics                                                          */
+/* ------------------------------------------------------------------ */
+
+static void fill_diagnostics(mofft_plan_diagnostics *d,
+                             const mofft_plan_common *p) {
+  if (!d) return;
+  d->radix_count = (size_t)p->s.depth;
+  memcpy(d->radices, p->s.factors, (size_t)p->s.depth * sizeof(int));
+  d->blocked_transpose_mask = p->s.pass_mask;
+  d->direct_input = p->direct_input;
+  d->stage_layout = p->stage_layout;
+  d->kernel_bank = p->kernel_bank;
+  d->kernel_bank_count = kernel_bank_count();
+  d->length = p->s.length;
+  d->precision_bits = p->s.precision_bits;
+  d->direction = (mofft_direction)p->direction;
+  d->placement = p->placement;
+}
+
+/* ------------------------------------------------------------------ */
+/* MEASURE planner (src/mofft.c:798-1052 restated for GPU schedules)    */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  int radices[MOFFT_MAX_PLAN_DEPTH];
+  size_t depth;
+  uint64_t mask;
+  double model;
+  double samples[MOFFT_MAX_TIMING_SAMPLES];
+  double median;
+} measured_candidate;
+
+static double scalar_median(double *v, size_t n) {
+  if (n == 0) return DBL_MAX;
+  for (size_t i = 1; i < n; ++i) {
+    const double x = v[i];
+    size_t j = i;
+    while (j > 0 && x < v[j - 1]) { v[j] = v[j - 1]; --j; }
+    v[j] = x;
+  }
+  return (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+static double schedule_model_cost(const mofft_schedule *s) {
+  /* HBM round trips dominate: one read + one write per pass, plus a small
+   * per-stage compute term */
+  const double bytes = 2.0 * (double)s->length * (double)s->batch *
+                       (double)complex_bytes(s->precision_bits);
+  return (double)s->pass_count * bytes / 8.0e12 +
+         (double)s->depth * 1e-7;
+}
+
+static int same_candidate(const measured_candidate *a,
+                          const measured_candidate *b) {
+  return a->depth == b->depth && a->mask == b->mask &&
+         memcmp(a->radices, b->radices, a->depth * sizeof(int)) == 0;
+}
+
+static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
+                          size_t length, size_t batch, int precision_bits,
+                          const int *radices, size_t depth, uint64_t mask) {
+  if (*count >= cap) return;
+  mofft_schedule s;
+  if (build_schedule(length, batch, precision_bits, radices, depth, mask, &s) !=
+      MOFFT_SUCCESS)
+    return;
+  measured_candidate c;
+  memset(&c, 0, sizeof(c));
+  memcpy(c.radices, radices, depth * sizeof(int));
+  c.depth = depth;
+  c.mask = s.pass_mask; /* effective grouping */
+  c.model = schedule_model_cost(&s);
+  for (size_t i = 0; i < *count; ++i)
+    if (same_candidate(&pool[i], &c)) return;
+  pool[(*count)++] = c;
+}
+
+static void enumerate_splits(int precision_bits, size_t remaining,
+                             pass_split *cur, int max_passes,
+                             measured_candidate *pool, size_t *count,
+                             size_t cap, size_t length, size_t batch) {
+  if (*count >= cap) return;
+  if (remaining == 1) {
+    int radices[MOFFT_MAX_PLAN_DEPTH];
+    size_t d = 0;
+    uint64_t mask = 0;
+    for (int p = cur->count - 1; p >= 0; --p) {
+      const mofft_registry_entry *e = &mofft_registry[cur->kernels[p]];
+      if (d + (size_t)e->stage_count > MOFFT_MAX_PLAN_DEPTH) return;
+      for (int i = e->stage_count - 1; i >= 0; --i) radices[d++] = e->stages[i];
+      if (p > 0) mask |= (uint64_t)1 << (d - 1);
+    }
+    add_candidate(pool, count, cap, length, batch, precision_bits, radices, d,
+                  mask);
+    return;
+  }
+  if (cur->count >= max_passes) return;
+  for (int i = 0; i < mofft_registry_count; ++i) {
+    const mofft_registry_entry *e = &mofft_registry[i];
+    if (e->precision_bits != precision_bits || e->rows) continue;
+    const size_t l = (size_t)e->length;
+    if (l < 2 || remaining % l) continue;
+    cur->lengths[cur->count] = l;
+    cur->kernels[cur->count] = i;
+    cur->count++;
+    enumerate_splits(precision_bits, remaining / l, cur, max_passes, pool,
+                     count, cap, length, batch);
+    cur->count--;
+  }
+}
+
+static int candidate_model_compare(const void *l, const void *r) {
+  const measured_candidate *a = (const measured_candidate *)l;
+  const measured_candidate *b = (const measured_candidate *)r;
+  if (a->model < b->model) return -1;
+  if (a->model > b->model) return 1;
+  if (a->depth != b->depth) return a->depth < b->depth ? -1 : 1;
+  for (size_t i = 0; i < a->depth; ++i)
+    if (a->radices[i] != b->radices[i]) return a->radices[i] < b->radices[i] ? -1 : 1;
+  if (a->mask != b->mask) return a->mask < b->mask ? -1 : 1;
+  return 0;
+}
+
+static int candidate_time_compare(const void *l, const void *r) {
+  const measured_candidate *a = (const measured_candidate *)l;
+  const measured_candidate *b = (const measured_candidate *)r;
+  if (a->median < b->median) return -1;
+  if (a->median > b->median) return 1;
+  return candidate_model_compare(l, r);
+}
+
+static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
+                                     size_t batch, int precision_bits,
+                                     int direction, mofft_placement placement,
+                                     const mofft_plan_options *options,
+                                     mofft_plan_diagnostics *diagnostics) {
+  const size_t cap = 4 * MOFFT_MAX_SEARCH_CANDIDATES;
+  measured_candidate *pool =
+      (measured_candidate *)calloc(cap, sizeof(measured_candidate));
+  if (!pool) return MOFFT_ALLOCATION_FAILURE;
+  size_t count = 0;
+  int radices[MOFFT_MAX_PLAN_DEPTH];
+  size_t depth = 0;
+  uint64_t mask = 0;
+  if (model_chain(length, precision_bits, radices, &depth, &mask) ==
+      MOFFT_SUCCESS)
+    add_candidate(pool, &count, cap, length, batch, precision_bits, radices,
+                  depth, mask);
+  pass_split cur;
+  memset(&cur, 0, sizeof(cur));
+  for (int max_passes = 1; max_passes <= 4 && count < cap; ++max_passes)
+    enumerate_splits(precision_bits, length, &cur, max_passes, pool, &count,
+                     cap, length, batch);
+  /* the all-blocked variant of the model chain: one pass per radix */
+  if (depth > 0)
+    add_candidate(pool, &count, cap, length, batch, precision_bits, radices,
+                  depth, depth > 1 ? (((uint64_t)1 << (depth - 1)) - 1) : 0);
+  if (count == 0) { free(pool); return MOFFT_UNSUPPORTED_SIZE; }
+  qsort(pool, count, sizeof(pool[0]), candidate_model_compare);
+  size_t selected = count < options->candidate_limit ? count
+                                                     : options->candidate_limit;
+  /* deterministic shuffle of the measurement order (src/mofft.c:929-942) */
+  uint32_t state = (uint32_t)length ^ ((uint32_t)precision_bits << 16) ^
+                   UINT32_C(0x9e3779b9);
+  for (size_t i = selected; i > 1; --i) {
+    state ^= state << 13; state ^= state >> 17; state ^= state << 5;
+    const size_t j = (size_t)state % i;
+    measured_candidate t = pool[i - 1]; pool[i - 1] = pool[j]; pool[j] = t;
+  }
+  const size_t cb = complex_bytes(precision_bits);
+  const size_t bytes = sat_mul(sat_mul(length, batch), cb);
+  void *din = NULL, *dout = NULL, *dref = NULL;
+  mofft_status st = MOFFT_SUCCESS;
+  if (mofft_hip_malloc(&din, bytes) || mofft_hip_malloc(&dout, bytes) ||
+      (placement == MOFFT_IN_PLACE && mofft_hip_malloc(&dref, bytes))) {
+    st = MOFFT_ALLOCATION_FAILURE;
+    goto done;
+  }
+  {
+    /* deterministic input (src/mofft.c:942-957) generated on the host */
+    const size_t host_elems = length * batch * 2;
+    void *host = malloc(bytes);
+    if (!host) { st = MOFFT_ALLOCATION_FAILURE; goto done; }
+    uint32_t x = UINT32_C(0x7f4a7c15);
+    for (size_t i = 0; i < host_elems; ++i) {
+      x ^= x << 13; x ^= x >> 17; x ^= x << 5;
+      const double v = ((double)(x & UINT32_C(0xffff)) / 32768.0) - 1.0;
+      if (precision_bits == 32) ((float *)host)[i] = (float)v;
+      else ((double *)host)[i] = v;
+    }
+    const int rc = mofft_hip_copy(din, host, bytes);
+    free(host);
+    if (rc) { st = MOFFT_INTERNAL_ERROR; goto done; }
+    if (dref && mofft_hip_copy(dref, din, bytes)) { st = MOFFT_INTERNAL_ERROR; goto done; }
+  }
+  {
+    const size_t iterations = length * batch < 65536 ? 8 : (length * batch < (1u << 22) ? 4 : 1);
+    for (size_t i = 0; i < selected; ++i) {
+      measured_candidate *c = &pool[i];
+      mofft_plan_common *p = NULL;
+      st = plan_from_chain(&p, length, batch, precision_bits, direction,
+                           placement, c->radices, c->depth, c->mask);
+      if (st != MOFFT_SUCCESS) goto done;
+      const void *xin = placement == MOFFT_IN_PLACE ? dout : din;
+      if (placement == MOFFT_IN_PLACE) mofft_hip_copy(dout, dref, bytes);
+      st = execute_device(p, xin, dout, p->workspace,
+                          schedule_workspace(&p->s, placement), NULL);
+      if (st == MOFFT_SUCCESS && mofft_hip_synchronize(NULL))
+        st = MOFFT_INTERNAL_ERROR;
+      for (uint32_t smp = 0; smp < options->timing_samples && st == MOFFT_SUCCESS; ++smp) {
+        double times[8];
+        for (size_t it = 0; it < iterations && st == MOFFT_SUCCESS; ++it) {
+          if (placement == MOFFT_IN_PLACE) {
+            mofft_hip_copy(dout, dref, bytes);
+            mofft_hip_synchronize(NULL);
+          }
+          const double t0 = monotonic_seconds();
+          st = execute_device(p, xin, dout, p->workspace,
+                              schedule_workspace(&p->s, placement), NULL);
+          if (st == MOFFT_SUCCESS && mofft_hip_synchronize(NULL))
+            st = MOFFT_INTERNAL_ERROR;
+          times[it] = monotonic_seconds() - t0;
+        }
+        c->samples[smp] = scalar_median(times, iterations);
+      }
+      plan_release(p);
+      if (st != MOFFT_SUCCESS) goto done;
+      c->median = scalar_median(c->samples, options->timing_samples);
+    }
+  }
+  qsort(pool, selected, sizeof(pool[0]), candidate_time_compare);
+  st = plan_from_chain(out, length, batch, precision_bits, direction, placement,
+                       pool[0].radices, pool[0].depth, pool[0].mask);
+  if (st == MOFFT_SUCCESS && diagnostics) {
+    diagnostics->measured = 1;
+    diagnostics->candidates_measured = selected;
+    diagnostics->selected_median_seconds = pool[0].median;
+    fill_diagnostics(diagnostics, *out);
+  }
+done:
+  mofft_hip_free(din);
+  mofft_hip_free(dout);
+  mofft_hip_free(dref);
+  free(pool);
+  return st;
+}
+
+/* ------------------------------------------------------------------ */
+/* plan creation entry points                                           */
+/* ------------------------------------------------------------------ */
+
+static mofft_status create_with_options(mofft_plan_common **out,
+                                        size_t length, size_t batch,
+                                        int precision_bits, int direction,
+                                        mofft_placement placement,
+                                        const mofft_plan_options *given,
+                                        mofft_plan_diagnostics *diagnostics) {
+  if (!out) return MOFFT_INVALID_ARGUMENT;
+  *out = NULL;
+  if (diagnostics) memset(diagnostics, 0, sizeof(*diagnostics));
+  if ((direction != MOFFT_FORWARD && direction != MOFFT_BACKWARD) ||
+      (placement != MOFFT_IN_PLACE && placement != MOFFT_OUT_OF_PLACE) ||
+      batch == 0)
+    return MOFFT_INVALID_ARGUMENT;
+  mofft_plan_options options;
+  mofft_status st = validate_options(given, &options);
+  if (st != MOFFT_SUCCESS) return st;
+  if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
+  const double start = monotonic_seconds();
+  if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
+  if (options.mode == MOFFT_PLANNER_MEASURE) {
+    st = search_best_plan(out, length, batch, precision_bits, direction,
+                          placement, &options, diagnostics);
+    if (st == MOFFT_SUCCESS && diagnostics)
+      diagnostics->planning_seconds = monotonic_seconds() - start;
+    return st;
+  }
+  int radices[MOFFT_MAX_PLAN_DEPTH];
+  size_t depth = 0;
+  uint64_t mask = 0;
+  int direct = 0, layout = 0;
+  unsigned int bank = 0;
+  const int w = wisdom_lookup(length, precision_bits, radices, &depth, &mask,
+                              &direct, &layout, &bank);
+  if (w < 0) return MOFFT_WISDOM_NOT_FOUND;
+  if (w == 0) {
+    if (options.flags & MOFFT_PLAN_STRICT_WISDOM) return MOFFT_WISDOM_NOT_FOUND;
+    st = model_chain(length, precision_bits, radices, &depth, &mask);
+    if (st != MOFFT_SUCCESS) return st;
+  }
+  st = plan_from_chain(out, length, batch, precision_bits, direction,
+                       placement, radices, depth, mask);
+  if (st != MOFFT_SUCCESS) return st;
+  if (w > 0) {
+    (*out)->direct_input = direct;
+    (*out)->stage_layout = (mofft_stage_layout)layout;
+    (*out)->kernel_bank = bank;
+  }
+  if (diagnostics) {
+    diagnostics->used_wisdom = w > 0;
+    diagnostics->used_model_fallback = w == 0;
+    fill_diagnostics(diagnostics, *out);
+    diagnostics->planning_seconds = monotonic_seconds() - start;
+  }
+  return MOFFT_SUCCESS;
+}
+
+static mofft_status create_with_radices(mofft_plan_common **out,
+                                        size_t length, size_t batch,
+                                        int precision_bits, int direction,
+                                        mofft_placement placement,
+                                        const int *radices, size_t count) {
+  if (!out || (direction != MOFFT_FORWARD && direction != MOFFT_BACKWARD) ||
+      (placement != MOFFT_IN_PLACE && placement != MOFFT_OUT_OF_PLACE) ||
+      batch == 0)
+    return MOFFT_INVALID_ARGUMENT;
+  *out = NULL;
+  mofft_status st = validate_chain(length, radices, count);
+  if (st != MOFFT_SUCCESS) return st;
+  return plan_from_chain(out, length, batch, precision_bits, direction,
+                         placement, radices, count, 0);
+}
+
+mofft_status mofft_plan_create_f32(mofft_plan_f32 **plan, size_t length,
+                                   mofft_direction direction,
+                                   mofft_placement placement) {
+  if (!plan) return MOFFT_INVALID_ARGUMENT;
+  return create_with_options((mofft_plan_common **)(void *)plan, length, 1, 32,
+                             direction, placement, NULL, NULL);
+}
+
+mofft_status mofft_plan_create_f64(mofft_plan_f64 **plan, size_t length,
+                                   mofft_direction direction,
+                                   mofft_placement placement) {
+  if (!plan) return MOFFT_INVALID_ARGUMENT;
+  return create_with_options((mofft_plan_common **)(void *)plan, length, 1, 64,
+                             direction, placement, NULL, NULL);
+}
+
+mofft_status mofft_plan_create_with_options_f32(
+    mofft_plan_f32 **plan, size_t length, mofft_direction direction,
+    mofft_placement placement, const mofft_plan_options *options,
+    mofft_plan_diagnostics *diagnostics) {
+  return create_with_options((mofft_plan_common **)(void *)plan, length, 1, 32,
+                             direction, placement, options, diagnostics);
+}
+
+mofft_status mofft_plan_create_with_options_f64(
+    mofft_plan_f64 **plan, size_t length, mofft_direction direction,
+    mofft_placement placement, const mofft_plan_options *options,
+    mofft_plan_diagnostics *diagnostics) {
+  return create_with_options((mofft_plan_common **)(void *)plan, length, 1, 64,
+                             direction, placement, options, diagnostics);
+}
+
+mofft_status mofft_plan_create_batched_f32(
+    mofft_plan_f32 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const mofft_plan_options *options, mofft_plan_diagnostics *diagnostics) {
+  return create_with_options((mofft_plan_common **)(void *)plan, length, batch,
+                             32, direction, placement, options, diagnostics);
+}
+
+mofft_status mofft_plan_create_batched_f64(
+    mofft_plan_f64 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement,
+    const mofft_plan_options *options, mofft_plan_diagnostics *diagnostics) {
+  return create_with_options((mofft_plan_common **)(void *)plan, length, batch,
+                             64, direction, placement, options, diagnostics);
+}
+
+mofft_status mofft_plan_create_with_radices_f32(
+    mofft_plan_f32 **plan, size_t length, mofft_direction direction,
+    mofft_placement placement, const int *radices, size_t radix_count) {
+  return create_with_radices((mofft_plan_common **)(void *)plan, length, 1, 32,
+                             direction, placement, radices, radix_count);
+}
+
+mofft_status mofft_plan_create_with_radices_f64(
+    mofft_plan_f64 **plan, size_t length, mofft_direction direction,
+    mofft_placement placement, const int *radices, size_t radix_count) {
+  return create_with_radices((mofft_plan_common **)(void *)plan, length, 1, 64,
+                             direction, placement, radices, radix_count);
+}
+
+mofft_status mofft_plan_create_batched_with_radices_f32(
+    mofft_plan_f32 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement, const int *radices,
+    size_t radix_count) {
+  return create_with_radices((mofft_plan_common **)(void *)plan, length, batch,
+                             32, direction, placement, radices, radix_count);
+}
+
+mofft_status mofft_plan_create_batched_with_radices_f64(
+    mofft_plan_f64 **plan, size_t length, size_t batch,
+    mofft_direction direction, mofft_placement placement, const int *radices,
+    size_t radix_count) {
+  return create_with_radices((mofft_plan_common **)(void *)plan, length, batch,
+                             64, direction, placement, radices, radix_count);
+}
+
+static mofft_plan_common *bound_plan(size_t length, const void *input,
+                                     void *output, mofft_direction direction,
+                                     uint32_t flags, int precision_bits) {
+  if (!input || !output) return NULL;
+  mofft_plan_options options;
+  if (!options_from_bound_flags(flags, &options)) return NULL;
+  mofft_plan_common *p = NULL;
+  const mofft_placement placement =
+      input == output ? MOFFT_IN_PLACE : MOFFT_OUT_OF_PLACE;
+  if (create_with_options(&p, length, 1, precision_bits, direction, placement,
+                          &options, NULL) != MOFFT_SUCCESS)
+    return NULL;
+  p->bound_input = input;
+  p->bound_output = output;
+  p->has_bound_arrays = 1;
+  return p;
+}
+
+mofft_plan_f32 *mofft_plan_dft_1d_f32(size_t length,
+                                      const mofft_complex_f32 *input,
+                                      mofft_complex_f32 *output,
+                                      mofft_direction direction,
+                                      uint32_t flags) {
+  return (mofft_plan_f32 *)bound_plan(length, input, output, direction, flags, 32);
+}
+
+mofft_plan_f64 *mofft_plan_dft_1d_f64(size_t length,
+                                      const mofft_complex_f64 *input,
+                                      mofft_complex_f64 *output,
+                                      mofft_direction direction,
+                                      uint32_t flags) {
+  return (mofft_plan_f64 *)bound_plan(length, input, output, direction, flags, 64);
+}
+
+mofft_status mofft_execute_plan_f32(const mofft_plan_f32 *plan) {
+  if (!plan || !plan->c.has_bound_arrays) return MOFFT_INVALID_ARGUMENT;
+  return execute_sync(&plan->c, plan->c.bound_input, plan->c.bound_output);
+}
+
+mofft_status mofft_execute_plan_f64(const mofft_plan_f64 *plan) {
+  if (!plan || !plan->c.has_bound_arrays) return MOFFT_INVALID_ARGUMENT;
+  return execute_sync(&plan->c, plan->c.bound_input, plan->c.bound_output);
+}
+
+mofft_status mofft_execute_f32(const mofft_plan_f32 *plan,
+                               const mofft_complex_f32 *input,
+                               mofft_complex_f32 *output) {
+  if (!plan) return MOFFT_INVALID_ARGUMENT;
+  return execute_sync(&plan->c, input, output);
+}
+
+mofft_status mofft_execute_f64(const mofft_plan_f64 *plan,
+                               const mofft_complex_f64 *input,
+                               mofft_complex_f64 *output) {
+  if (!plan) return MOFFT_INVALID_ARGUMENT;
+  return execute_sync(&plan->c, input, output);
+}
+
+static mofft_status execute_async(const mofft_plan_common *p, const void *in,
+                                  void *out, void *work, size_t work_bytes,
+                                  int own_workspace, void *stream) {
+  if (!p || !in || !out) return MOFFT_INVALID_ARGUMENT;
+  if ((p->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_is_device_pointer(in) || !mofft_hip_is_device_pointer(out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (own_workspace) {
+    work = p->workspace;
+    work_bytes = schedule_workspace(&p->s, p->placement);
+  }
+  return execute_device(p, in, out, work, work_bytes, stream);
+}
+
+mofft_status mofft_execute_async_f32(const mofft_plan_f32 *plan,
+                                     const mofft_complex_f32 *input,
+                                     mofft_complex_f32 *output, void *stream) {
+  return execute_async(plan ? &plan->c : NULL, input, output, NULL, 0, 1, stream);
+}
+
+mofft_status mofft_execute_async_f64(const mofft_plan_f64 *plan,
+                                     const mofft_complex_f64 *input,
+                                     mofft_complex_f64 *output, void *stream) {
+  return execute_async(plan ? &plan->c : NULL, input, output, NULL, 0, 1, stream);
+}
+
+mofft_status mofft_plan_workspace_size_f32(const mofft_plan_f32 *plan,
+                                           size_t *bytes) {
+  if (!plan || !bytes) return MOFFT_INVALID_ARGUMENT;
+  *bytes = schedule_workspace(&plan->c.s, plan->c.placement);
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_workspace_size_f64(const mofft_plan_f64 *plan,
+                                           size_t *bytes) {
+  if (!plan || !bytes) return MOFFT_INVALID_ARGUMENT;
+  *bytes = schedule_workspace(&plan->c.s, plan->c.placement);
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_execute_async_workspace_f32(
+    const mofft_plan_f32 *plan, const mofft_complex_f32 *input,
+    mofft_complex_f32 *output, void *workspace, size_t workspace_bytes,
+    void *stream) {
+  return execute_async(plan ? &plan->c : NULL, input, output, workspace,
+                       workspace_bytes, 0, stream);
+}
+
+mofft_status mofft_execute_async_workspace_f64(
+    const mofft_plan_f64 *plan, const mofft_complex_f64 *input,
+    mofft_complex_f64 *output, void *workspace, size_t workspace_bytes,
+    void *stream) {
+  return execute_async(plan ? &plan->c : NULL, input, output, workspace,
+                       workspace_bytes, 0, stream);
+}
+
+mofft_status mofft_plan_get_batch_f32(const mofft_plan_f32 *plan,
+                                      size_t *batch) {
+  if (!plan || !batch) return MOFFT_INVALID_ARGUMENT;
+  *batch = plan->c.s.batch;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_get_batch_f64(const mofft_plan_f64 *plan,
+                                      size_t *batch) {
+  if (!plan || !batch) return MOFFT_INVALID_ARGUMENT;
+  *batch = plan->c.s.batch;
+  return MOFFT_SUCCESS;
+}
+
+void mofft_plan_destroy_f32(mofft_plan_f32 *plan) {
+  plan_release(plan ? &plan->c : NULL);
+}
+
+void mofft_plan_destroy_f64(mofft_plan_f64 *plan) {
+  plan_release(plan ? &plan->c : NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* plan description (host only)                                         */
+/* ------------------------------------------------------------------ */
+
+static void describe_schedule(const mofft_schedule *s,
+                              mofft_placement placement,
+                              mofft_plan_description *d) {
+  memset(d, 0, sizeof(*d));
+  d->length = s->length;
+  d->batch = s->batch;
+  d->precision_bits = s->precision_bits;
+  d->radix_count = (size_t)s->depth;
+  memcpy(d->radices, s->factors, (size_t)s->depth * sizeof(int));
+  d->pass_mask = s->pass_mask;
+  d->pass_count = s->pass_count;
+  for (int i = 0; i < s->pass_count && i < MOFFT_MAX_PASSES; ++i) {
+    d->pass_length[i] = s->passes[i].length;
+    d->pass_stride[i] = s->passes[i].stride;
+    d->pass_span[i] = s->passes[i].span;
+    d->pass_kernel[i] = s->passes[i].kernel;
+  }
+  d->chunk_transforms = s->chunk;
+  d->workspace_bytes = schedule_workspace(s, placement);
+  d->twiddle_bytes = s->twiddle_count * complex_bytes(s->precision_bits);
+}
+
+mofft_status mofft_plan_describe(size_t length, size_t batch,
+                                 int precision_bits, const int *radices,
+                                 size_t radix_count, uint64_t pass_mask,
+                                 mofft_plan_description *description) {
+  if (!description || batch == 0 ||
+      (precision_bits != 32 && precision_bits != 64))
+    return MOFFT_INVALID_ARGUMENT;
+  int chain[MOFFT_MAX_PLAN_DEPTH];
+  size_t depth = radix_count;
+  uint64_t mask = pass_mask;
+  if (!radices) {
+    if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
+    size_t d2 = 0;
+    int direct = 0, layout = 0;
+    unsigned int bank = 0;
+    const int w = wisdom_lookup(length, precision_bits, chain, &d2, &mask,
+                                &direct, &layout, &bank);
+    if (w < 0) return MOFFT_WISDOM_NOT_FOUND;
+    if (w == 0) {
+      mofft_status st = model_chain(length, precision_bits, chain, &d2, &mask);
+      if (st != MOFFT_SUCCESS) return st;
+    }
+    depth = d2;
+  } else {
+    if (radix_count == 0 || radix_count > MOFFT_MAX_PLAN_DEPTH)
+      return MOFFT_INVALID_ARGUMENT;
+    memcpy(chain, radices, radix_count * sizeof(int));
+  }
+  mofft_schedule s;
+  mofft_status st = build_schedule(length, batch, precision_bits, chain, depth,
+                                   mask, &s);
+  if (st != MOFFT_SUCCESS) return st;
+  describe_schedule(&s, MOFFT_IN_PLACE, description);
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_get_description_f32(const mofft_plan_f32 *plan,
+                                            mofft_plan_description *out) {
+  if (!plan || !out) return MOFFT_INVALID_ARGUMENT;
+  describe_schedule(&plan->c.s, plan->c.placement, out);
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
+                                            mofft_plan_description *out) {
+  if (!plan || !out) return MOFFT_INVALID_ARGUMENT;
+  describe_schedule(&plan->c.s, plan->c.placement, out);
+  return MOFFT_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* expert setters (include/mofft.h:146-176)                             */
+/* ------------------------------------------------------------------ */
+
+/* Changing the pass grouping rebuilds the schedule and twiddle tables. */
+static mofft_status regroup(mofft_plan_common *p, uint64_t mask) {
+  mofft_schedule s;
+  mofft_status st = build_schedule(p->s.length, p->s.batch,
+                                   p->s.precision_bits, p->s.factors,
+                                   (size_t)p->s.depth, mask, &s);
+  if (st != MOFFT_SUCCESS) return st;
+  if (s.pass_mask == p->s.pass_mask) return MOFFT_SUCCESS;
+  mofft_plan_common tmp = *p;
+  tmp.s = s;
+  tmp.twiddles = NULL;
+  tmp.workspace = NULL;
+  st = fill_twiddles(&tmp);
+  if (st == MOFFT_SUCCESS) {
+    const size_t ws = schedule_workspace(&s, p->placement);
+    if (ws && mofft_hip_malloc(&tmp.workspace, ws) != 0)
+      st = MOFFT_ALLOCATION_FAILURE;
+  }
+  if (st != MOFFT_SUCCESS) {
+    mofft_hip_free(tmp.twiddles);
+    mofft_hip_free(tmp.workspace);
+    return st;
+  }
+  mofft_hip_free(p->twiddles);
+  mofft_hip_free(p->workspace);
+  p->s = s;
+  p->twiddles = tmp.twiddles;
+  p->workspace = tmp.workspace;
+  return MOFFT_SUCCESS;
+}
+
+static mofft_status set_mask(mofft_plan_common *p, uint64_t mask) {
+  if (!p) return MOFFT_INVALID_ARGUMENT;
+  const int transpose_count = p->s.depth > 1 ? p->s.depth - 1 : 0;
+  if (transpose_count < 64 && (mask >> transpose_count)) return MOFFT_INVALID_ARGUMENT;
+  return regroup(p, mask);
+}
+
+static mofft_status set_strategy(mofft_plan_common *p,
+                                 mofft_transpose_strategy strategy) {
+  if (!p || (strategy != MOFFT_TRANSPOSE_LINEAR &&
+             strategy != MOFFT_TRANSPOSE_BLOCKED))
+    return MOFFT_INVALID_ARGUMENT;
+  const uint64_t all = p->s.depth <= 1 ? 0
+                                       : ((uint64_t)1 << (p->s.depth - 1)) - 1;
+  return regroup(p, strategy == MOFFT_TRANSPOSE_BLOCKED ? all : 0);
+}
+
+mofft_status mofft_plan_set_transpose_strategy_f32(
+    mofft_plan_f32 *plan, mofft_transpose_strategy strategy) {
+  return set_strategy(plan ? &plan->c : NULL, strategy);
+}
+
+mofft_status mofft_plan_set_transpose_strategy_f64(
+    mofft_plan_f64 *plan, mofft_transpose_strategy strategy) {
+  return set_strategy(plan ? &plan->c : NULL, strategy);
+}
+
+mofft_status mofft_plan_set_blocked_transpose_mask_f32(mofft_plan_f32 *plan,
+                                                       uint64_t levels) {
+  return set_mask(plan ? &plan->c : NULL, levels);
+}
+
+mofft_status mofft_plan_set_blocked_transpose_mask_f64(mofft_plan_f64 *plan,
+                                                       uint64_t levels) {
+  return set_mask(plan ? &plan->c : NULL, levels);
+}
+
+/* direct_input and stage_layout select CPU data layouts in the reference
+ * (src/mofft.c:1946-2009); on the GPU both layouts are the same LDS pass, so
+ * the values are validated, kept, reported and exported only. */
+mofft_status mofft_plan_set_direct_input_f32(mofft_plan_f32 *plan, int enabled) {
+  if (!plan || (enabled != 0 && enabled != 1)) return MOFFT_INVALID_ARGUMENT;
+  plan->c.direct_input = enabled;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_set_direct_input_f64(mofft_plan_f64 *plan, int enabled) {
+  if (!plan || (enabled != 0 && enabled != 1)) return MOFFT_INVALID_ARGUMENT;
+  plan->c.direct_input = enabled;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_set_stage_layout_f32(mofft_plan_f32 *plan,
+                                             mofft_stage_layout layout) {
+  if (!plan || (layout != MOFFT_STAGE_LAYOUT_TRANSPOSE &&
+                layout != MOFFT_STAGE_LAYOUT_SECTION))
+    return MOFFT_INVALID_ARGUMENT;
+  plan->c.stage_layout = layout;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_set_stage_layout_f64(mofft_plan_f64 *plan,
+                                             mofft_stage_layout layout) {
+  if (!plan || (layout != MOFFT_STAGE_LAYOUT_TRANSPOSE &&
+                layout != MOFFT_STAGE_LAYOUT_SECTION))
+    return MOFFT_INVALID_ARGUMENT;
+  plan->c.stage_layout = layout;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_set_kernel_bank_f32(mofft_plan_f32 *plan,
+                                            unsigned int bank) {
+  if (!plan || bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
+  plan->c.kernel_bank = bank;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_set_kernel_bank_f64(mofft_plan_f64 *plan,
+                                            unsigned int bank) {
+  if (!plan || bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
+  plan->c.kernel_bank = bank;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_get_kernel_bank_f32(const mofft_plan_f32 *plan,
+                                            unsigned int *bank) {
+  if (!plan || !bank) return MOFFT_INVALID_ARGUMENT;
+  *bank = plan->c.kernel_bank;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_get_kernel_bank_f64(const mofft_plan_f64 *plan,
+                                            unsigned int *bank) {
+  if (!plan || !bank) return MOFFT_INVALID_ARGUMENT;
+  *bank = plan->c.kernel_bank;
+  return MOFFT_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ */
+/* plan recipes (format of src/mofft.c:1769-1789, parser rules 1675-1758) */
+/* ------------------------------------------------------------------ */
+
+const char *mofft_build_id(void) { return MOFFT_BINARY_ID; }
+
+static int recipe_id_valid(const char *id) {
+  if (!id || strlen(id) != 64) return 0;
+  for (size_t i = 0; i < 64; ++i)
+    if (!((id[i] >= '0' && id[i] <= '9') || (id[i] >= 'a' && id[i] <= 'f')))
+      return 0;
+  return 1;
+}
+
+static int device_family(uint32_t *family) {
+  if (!mofft_hip_device_available()) return 0;
+  *family = MOFFT_GPU_FAMILY;
+  return 1;
+}
+
+static int parse_uint(const char *token, int base, uint64_t *value) {
+  if (!token || !*token) return 0;
+  uint64_t v = 0;
+  for (const char *c = token; *c; ++c) {
+    int digit;
+    if (*c >= '0' && *c <= '9') digit = *c - '0';
+    else if (base == 16 && *c >= 'a' && *c <= 'f') digit = *c - 'a' + 10;
+    else if (base == 16 && *c >= 'A' && *c <= 'F') digit = *c - 'A' + 10;
+    else return 0;
+    if (v > (UINT64_MAX - (uint64_t)digit) / (uint64_t)base) return 0;
+    v = v * (uint64_t)base + (uint64_t)digit;
+  }
+  *value = v;
+  return 1;
+}
+
+typedef struct {
+  size_t length;
+  int direction;
+  mofft_placement placement;
+  unsigned int bank;
+  uint64_t mask;
+  int direct_input;
+  mofft_stage_layout layout;
+  size_t depth;
+  int radices[MOFFT_MAX_PLAN_DEPTH];
+} recipe_fields;
+
+static mofft_status parse_recipe(const char *recipe, size_t size,
+                                 int precision_bits, recipe_fields *f) {
+  if (!recipe || size < 2 || size > MOFFT_RECIPE_MAX_BYTES ||
+      recipe[size - 1] != '\0' || recipe[size - 2] != '\n' ||
+      memchr(recipe, '\0', size - 1) != NULL)
+    return MOFFT_INVALID_ARGUMENT;
+  char text[MOFFT_RECIPE_MAX_BYTES];
+  for (size_t i = 0; i + 2 < size; ++i) {
+    const unsigned char ch = (unsigned char)recipe[i];
+    if (ch < 32 || ch > 126) return MOFFT_INVALID_ARGUMENT;
+    text[i] = (char)ch;
+  }
+  text[size - 2] = '\0';
+  if (size > 2 && (text[0] == ' ' || (size > 3 && text[size - 3] == ' ')))
+    return MOFFT_INVALID_ARGUMENT;
+  char *tokens[12 + MOFFT_MAX_PLAN_DEPTH];
+  size_t n = 0;
+  char *cur = text;
+  while (*cur) {
+    if (n == 12 + MOFFT_MAX_PLAN_DEPTH) return MOFFT_INVALID_ARGUMENT;
+    tokens[n++] = cur;
+    while (*cur && *cur != ' ') ++cur;
+    if (*cur == ' ') {
+      *cur++ = '\0';
+      if (*cur == ' ') return MOFFT_INVALID_ARGUMENT;
+    }
+  }
+  if (n < 13 || strcmp(tokens[0], "MOFFT_PLAN_V1") != 0 ||
+      !recipe_id_valid(MOFFT_BINARY_ID) || strlen(tokens[1]) != 64 ||
+      strcmp(tokens[1], MOFFT_BINARY_ID) != 0)
+    return MOFFT_INCOMPATIBLE_RECIPE;
+  if (strlen(tokens[2]) != 8 || strlen(tokens[8]) != 16)
+    return MOFFT_INVALID_ARGUMENT;
+  uint64_t family, length, placement, bank, mask, direct, layout, depth;
+  if (!parse_uint(tokens[2], 16, &family) || family > UINT32_MAX ||
+      !parse_uint(tokens[4], 10, &length) ||
+      !parse_uint(tokens[6], 10, &placement) ||
+      !parse_uint(tokens[7], 10, &bank) ||
+      !parse_uint(tokens[8], 16, &mask) ||
+      !parse_uint(tokens[9], 10, &direct) ||
+      !parse_uint(tokens[10], 10, &layout) ||
+      !parse_uint(tokens[11], 10, &depth))
+    return MOFFT_INVALID_ARGUMENT;
+  if ((precision_bits == 32 && strcmp(tokens[3], "f32") != 0) ||
+      (precision_bits == 64 && strcmp(tokens[3], "f64") != 0))
+    return MOFFT_INCOMPATIBLE_RECIPE;
+  if ((strcmp(tokens[5], "-1") != 0 && strcmp(tokens[5], "1") != 0) ||
+      (uint64_t)(size_t)length != length || length < 2 || placement > 1 ||
+      bank >= kernel_bank_count() || direct > 1 || layout > 1 || depth == 0 ||
+      depth > MOFFT_MAX_PLAN_DEPTH || n != 12 + (size_t)depth)
+    return MOFFT_INVALID_ARGUMENT;
+  if (depth - 1 < 64 && (mask >> (depth - 1)) != 0) return MOFFT_INVALID_ARGUMENT;
+  f->length = (size_t)length;
+  f->direction = strcmp(tokens[5], "-1") == 0 ? MOFFT_FORWARD : MOFFT_BACKWARD;
+  f->placement = (mofft_placement)placement;
+  f->bank = (unsigned int)bank;
+  f->mask = mask;
+  f->direct_input = (int)direct;
+  f->layout = (mofft_stage_layout)layout;
+  f->depth = (size_t)depth;
+  size_t remaining = f->length;
+  for (size_t i = 0; i < f->depth; ++i) {
+    uint64_t r = 0;
+    if (!parse_uint(tokens[12 + i], 10, &r) || r > INT32_MAX || r < 2 ||
+        remaining % (size_t)r != 0)
+      return MOFFT_INVALID_ARGUMENT;
+    f->radices[i] = (int)r;
+    remaining /= (size_t)r;
+  }
+  if (remaining != 1) return MOFFT_INVALID_ARGUMENT;
+  uint32_t current = 0;
+  if (!device_family(&current) || current != (uint32_t)family)
+    return MOFFT_INCOMPATIBLE_RECIPE;
+  return MOFFT_SUCCESS;
+}
+
+static mofft_status export_plan(const mofft_plan_common *p, int precision_bits,
+                                char *buffer, size_t capacity,
+                                size_t *required) {
+  if (!p || !required) return MOFFT_INVALID_ARGUMENT;
+  uint32_t family = 0;
+  if (!device_family(&family) || !recipe_id_valid(MOFFT_BINARY_ID))
+    return MOFFT_UNSUPPORTED_HARDWARE;
+  char recipe[MOFFT_RECIPE_MAX_BYTES];
+  int written = snprintf(recipe, sizeof(recipe),
+                         "MOFFT_PLAN_V1 %s %08x f%d %zu %d %d %u %016llx %d %d %d",
+                         MOFFT_BINARY_ID, family, precision_bits, p->s.length,
+                         p->direction, (int)p->placement, p->kernel_bank,
+                         (unsigned long long)p->s.pass_mask, p->direct_input,
+                         (int)p->stage_layout, p->s.depth);
+  if (written < 0 || (size_t)written >= sizeof(recipe)) return MOFFT_INTERNAL_ERROR;
+  size_t used = (size_t)written;
+  for (int i = 0; i < p->s.depth; ++i) {
+    written = snprintf(recipe + used, sizeof(recipe) - used, " %d", p->s.factors[i]);
+    if (written < 0 || (size_t)written >= sizeof(recipe) - used)
+      return MOFFT_INTERNAL_ERROR;
+    used += (size_t)written;
+  }
+  if (used + 2 > sizeof(recipe)) return MOFFT_INTERNAL_ERROR;
+  recipe[used++] = '\n';
+  recipe[used] = '\0';
+  *required = used + 1;
+  if (!buffer) return capacity == 0 ? MOFFT_SUCCESS : MOFFT_INVALID_ARGUMENT;
+  if (capacity < *required) return MOFFT_BUFFER_TOO_SMALL;
+  memcpy(buffer, recipe, *required);
+  return MOFFT_SUCCESS;
+}
+
+static mofft_status import_plan(mofft_plan_common **out, const char *recipe,
+                                size_t size, int precision_bits,
+                                mofft_plan_diagnostics *diagnostics) {
+  if (!out) return MOFFT_INVALID_ARGUMENT;
+  *out = NULL;
+  if (diagnostics) memset(diagnostics, 0, sizeof(*diagnostics));
+  recipe_fields f;
+  mofft_status st = parse_recipe(recipe, size, precision_bits, &f);
+  if (st != MOFFT_SUCCESS) return st;
+  if (validate_chain(f.length, f.radices, f.depth) != MOFFT_SUCCESS)
+    return MOFFT_INVALID_ARGUMENT;
+  st = plan_from_chain(out, f.length, 1, precision_bits, f.direction,
+                       f.placement, f.radices, f.depth, f.mask);
+  if (st != MOFFT_SUCCESS) { *out = NULL; return st; }
+  (*out)->direct_input = f.direct_input;
+  (*out)->stage_layout = f.layout;
+  (*out)->kernel_bank = f.bank;
+  if (diagnostics) {
+    diagnostics->used_wisdom = 1;
+    fill_diagnostics(diagnostics, *out);
+  }
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_export_f32(const mofft_plan_f32 *plan, char *buffer,
+                                   size_t capacity, size_t *required) {
+  return export_plan(plan ? &plan->c : NULL, 32, buffer, capacity, required);
+}
+
+mofft_status mofft_plan_export_f64(const mofft_plan_f64 *plan, char *buffer,
+                                   size_t capacity, size_t *required) {
+  return export_plan(plan ? &plan->c : NULL, 64, buffer, capacity, required);
+}
+
+mofft_status mofft_plan_import_f32(mofft_plan_f32 **plan, const char *recipe,
+                                   size_t recipe_size,
+                                   mofft_plan_diagnostics *diagnostics) {
+  return import_plan((mofft_plan_common **)(void *)plan, recipe, recipe_size,
+                     32, diagnostics);
+}
+
+mofft_status mofft_plan_import_f64(mofft_plan_f64 **plan, const char *recipe,
+                                   size_t recipe_size,
+                                   mofft_plan_diagnostics *diagnostics) {
+  return import_plan((mofft_plan_common **)(void *)plan, recipe, recipe_size,
+                     64, diagnostics);
+}
+
+/* ------------------------------------------------------------------ */
+/* analytical cost model (src/mofft.c:133-325) re-fitted to MI355X       */
+/* ------------------------------------------------------------------ */
+
+/* Cost unit: nanoseconds on one MI355X.  A stage's kernel cost is the
+ * codelet flop count of one butterfly (plus the 6(r-1) twiddle flops of a
+ * non-first stage) over `lanes` butterflies at the FP32 (157.3 TFLOP/s) or
+ * FP64 (78.6 TFLOP/s) vector peak.  Memory tiers: LDS 160 KiB/CU, L2 4 MiB
+ * per XCD, Infinity Cache 256 MiB, HBM3E 8 TB/s. */
+#define MODEL_LANES 64
+#define MODEL_CACHE_LINE 128
+
+static double codelet_flops(int radix) {
+  switch (radix) {
+    case 2: return 4; case 3: return 16; case 4: return 16; case 5: return 48;
+    case 6: return 57; case 7: return 96; case 8: return 56; case 9: return 121;
+    case 10: return 142; case 11: return 240; case 12: return 138;
+    case 13: return 336; case 14: return 259; case 15: return 276;
+    case 16: return 169; case 32: return 470; case 64: return 1192;
+    default: return -1;
+  }
+}
+
+double mofft_estimated_radix_cost(int radix, int precision_bits,
+                                  int first_stage) {
+  const double f = codelet_flops(radix);
+  if (f < 0 || (precision_bits != 32 && precision_bits != 64)) return 1.0e300;
+  const double flops = f + (first_stage ? 0.0 : 6.0 * (radix - 1));
+  const double peak = precision_bits == 32 ? 157.3e3 : 78.6e3; /* flop/ns */
+  return flops * MODEL_LANES / peak;
+}
+
+static double model_memory_cost(size_t working_set, size_t read, size_t write) {
+  static const struct { size_t cap; double read_bw, write_bw, mixed_bw, lat; } tier[] = {
+      {(size_t)160 << 10, 81920.0, 81920.0, 81920.0, 0.05},   /* LDS, all CUs */
+      {(size_t)32 << 20, 16000.0, 16000.0, 16000.0, 0.5},     /* 8 x 4 MiB L2 */
+      {(size_t)256 << 20, 10000.0, 10000.0, 10000.0, 1.0},    /* Infinity Cache */
+      {SIZE_MAX, 8000.0, 8000.0, 8000.0, 2.0},                /* HBM3E, B/ns */
+  };
+  for (size_t i = 0; i < sizeof(tier) / sizeof(tier[0]); ++i) {
+    if (working_set <= tier[i].cap) {
+      const double r = (double)read / tier[i].read_bw;
+      const double w = (double)write / tier[i].write_bw;
+      const double m = (double)(read + write) / tier[i].mixed_bw;
+      double c = r > w ? r : w;
+      if (m > c) c = m;
+      return c + tier[i].lat;
+    }
+  }
+  return 1.0e300;
+}
+
+typedef struct {
+  double kernel_cost, memory_cost, total_cost;
+  size_t read, write, ws;
+} stage_cost;
+
+static int estimate_stage(int radix, int precision_bits, int first_stage,
+                          size_t transform_length, size_t stage_length,
+                          size_t extra_read, stage_cost *r) {
+  if (radix < 2 || (precision_bits != 32 && precision_bits != 64) ||
+      transform_length == 0 || stage_length == 0 ||
+      stage_length > transform_length ||
+      transform_length % (size_t)radix != 0 || !r)
+    return 0;
+  const double rc = mofft_estimated_radix_cost(radix, precision_bits, first_stage);
+  if (rc >= 1.0e300) return 0;
+  const size_t independent = transform_length / (size_t)radix;
+  r->kernel_cost = rc * (double)((independent + MODEL_LANES - 1) / MODEL_LANES);
+  const size_t cb = complex_bytes(precision_bits);
+  const size_t data = sat_mul(transform_length, cb);
+  const size_t tw = first_stage ? 0 : sat_mul(stage_length, cb);
+  r->read = sat_add(data, tw);
+  r->write = data;
+  r->ws = sat_add(sat_mul(data, 2), tw);
+  r->memory_cost = model_memory_cost(r->ws, sat_add(r->read, extra_read), r->write);
+  r->total_cost = r->kernel_cost > r->memory_cost ? r->kernel_cost : r->memory_cost;
+  return 1;
+}
+
+double mofft_estimated_stage_cost(int radix, int precision_bits,
+                                  int first_stage, size_t transform_length,
+                                  size_t stage_length) {
+  stage_cost c;
+  return estimate_stage(radix, precision_bits, first_stage, transform_length,
+                        stage_length, 0, &c) ? c.total_cost : DBL_MAX;
+}
+
+static size_t overfetch(size_t bytes, size_t useful, size_t line) {
+  if (useful >= line) return bytes;
+  if (useful == 0 || bytes > SIZE_MAX / line) return SIZE_MAX;
+  const size_t product = bytes * line;
+  if (product > SIZE_MAX - (useful - 1)) return SIZE_MAX;
+  return (product + useful - 1) / useful;
+}
+
+/* Same accounting as src/mofft.c:190-304 with the reference's default build
+ * options (first-transpose fusion and repeated-twiddle broadcast on): the
+ * logical/transferred byte fields are layout invariants of the reference
+ * executor, the costs use the MI355X tiers above. */
+mofft_status mofft_estimated_plan_cost_breakdown_with_layout(
+    size_t length, int precision_bits, const int *radices, size_t radix_count,
+    uint64_t blocked_transpose_mask, int direct_input,
+    mofft_stage_layout layout, mofft_plan_cost_breakdown *breakdown) {
+  if (!radices || radix_count == 0 || radix_count > MOFFT_MAX_PLAN_DEPTH ||
+      length < 2 || (precision_bits != 32 && precision_bits != 64) ||
+      (direct_input != 0 && direct_input != 1) ||
+      (layout != MOFFT_STAGE_LAYOUT_TRANSPOSE &&
+       layout != MOFFT_STAGE_LAYOUT_SECTION) ||
+      blocked_transpose_mask >> (radix_count - 1) || !breakdown)
+    return MOFFT_INVALID_ARGUMENT;
+  memset(breakdown, 0, sizeof(*breakdown));
+  const size_t lanes = precision_bits == 32 ? 16 : 8;
+  const size_t cb = complex_bytes(precision_bits);
+  const size_t data = sat_mul(length, cb);
+  size_t n = length;
+  for (size_t level = 0; level < radix_count; ++level) {
+    const int radix = radices[level];
+    if (!is_supported_radix(radix) || n % (size_t)radix != 0)
+      return MOFFT_INVALID_ARGUMENT;
+    const size_t rem = n / (size_t)radix;
+    size_t extra = 0;
+    const int fused = radix_count == 2 && level == 0;
+    if (rem != 1 && layout == MOFFT_STAGE_LAYOUT_SECTION) {
+      const size_t t = overfetch(data, sat_mul(rem, cb), MODEL_CACHE_LINE);
+      extra = t > data ? t - data : 0;
+    } else if (rem != 1 && !fused) {
+      const size_t parent = length / n;
+      if (direct_input && parent >= lanes / 2) {
+        const size_t t = overfetch(data, sat_mul(parent, cb), MODEL_CACHE_LINE);
+        extra = t > data ? t - data : 0;
+      } else {
+        const int blocked = (int)((blocked_transpose_mask >> level) & 1u);
+        const size_t ws = sat_mul(data, 2);
+        const size_t reads = blocked ? data
+                                     : overfetch(data, sat_mul(parent, cb),
+                                                 MODEL_CACHE_LINE);
+        breakdown->layout_cost += model_memory_cost(ws, reads, data);
+        breakdown->logical_read_bytes = sat_add(breakdown->logical_read_bytes, data);
+        breakdown->logical_write_bytes = sat_add(breakdown->logical_write_bytes, data);
+        breakdown->transferred_read_bytes = sat_add(breakdown->transferred_read_bytes, reads);
+        breakdown->transferred_write_bytes = sat_add(breakdown->transferred_write_bytes, data);
+        if (ws > breakdown->peak_working_set_bytes) breakdown->peak_working_set_bytes = ws;
+      }
+    }
+    stage_cost sc;
+    if (!estimate_stage(radix, precision_bits, rem == 1, length, n, extra, &sc))
+      return MOFFT_INVALID_ARGUMENT;
+    breakdown->kernel_cost += sc.kernel_cost;
+    breakdown->memory_cost += sc.memory_cost;
+    breakdown->total_cost += sc.total_cost;
+    breakdown->logical_read_bytes = sat_add(breakdown->logical_read_bytes, sc.read);
+    breakdown->logical_write_bytes = sat_add(breakdown->logical_write_bytes, sc.write);
+    breakdown->transferred_read_bytes = sat_add(breakdown->transferred_read_bytes,
+                                                sat_add(sc.read, extra));
+    breakdown->transferred_write_bytes = sat_add(breakdown->transferred_write_bytes, sc.write);
+    if (sc.ws > breakdown->peak_working_set_bytes) breakdown->peak_working_set_bytes = sc.ws;
+    n = rem;
+  }
+  if (n != 1) return MOFFT_INVALID_ARGUMENT;
+  breakdown->total_cost += breakdown->layout_cost;
+  return MOFFT_SUCCESS;
+}
+
+double mofft_estimated_plan_cost_with_layout(
+    size_t length, int precision_bits, const int *radices, size_t radix_count,
+    uint64_t blocked_transpose_mask, int direct_input,
+    mofft_stage_layout layout) {
+  mofft_plan_cost_breakdown b;
+  return mofft_estimated_plan_cost_breakdown_with_layout(
+             length, precision_bits, radices, radix_count,
+             blocked_transpose_mask, direct_input, layout, &b) == MOFFT_SUCCESS
+             ? b.total_cost : DBL_MAX;
+}
+
+double mofft_estimated_plan_cost(size_t length, int precision_bits,
+                                 const int *radices, size_t radix_count,
+                                 uint64_t blocked_transpose_mask,
+                                 int direct_input) {
+  return mofft_estimated_plan_cost_with_layout(
+      length, precision_bits, radices, radix_count, blocked_transpose_mask,
+      direct_input, MOFFT_STAGE_LAYOUT_TRANSPOSE);
+}
+
+/* ------------------------------------------------------------------ */
+/* strings                                                              */
+/* ------------------------------------------------------------------ */
+
+const char *mofft_status_string(mofft_status status) {
+  switch (status) {
+    case MOFFT_SUCCESS: return "success";
+    case MOFFT_INVALID_ARGUMENT: return "invalid argument";
+    case MOFFT_UNSUPPORTED_SIZE: return "unsupported size";
+    case MOFFT_ALLOCATION_FAILURE: return "allocation failure";
+    case MOFFT_UNSUPPORTED_HARDWARE: return "unsupported hardware";
+    case MOFFT_INTERNAL_ERROR: return "internal error";
+    case MOFFT_WISDOM_NOT_FOUND: return "compiled wisdom not found";
+    case MOFFT_INCOMPATIBLE_RECIPE: return "incompatible plan recipe";
+    case MOFFT_BUFFER_TOO_SMALL: return "output buffer too small";
+    default: return "unknown status";
+  }
+}
+
+const char *mofft_version(void) { return "0.4.0"; }

@@ -1,0 +1,83 @@
+/*
+ * mofft_abi.h - the thin C ABI between the C runtime (mofft.c) and the HIP
+ * layer (hip/device.hip + generated kernels).
+ *
+ * This is the MI355X replacement for the reference's streaming-mode entry
+ * (src/streaming_entry.S:4-38) and generated dispatchers
+ * (compiler/mofft_compiler/emitter.py:557-620): the runtime decides the
+ * schedule on the host and calls these plain-C functions to allocate device
+ * memory, fill twiddle tables and launch pass kernels.  No HIP or C++ type
+ * crosses this boundary; streams travel as void *.
+ */
+#ifndef MOFFT_ABI_H
+#define MOFFT_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arguments of one pass-kernel launch.  A pass computes, for every column
+ * (b, j) with b < transforms and j < stride (= n / L), the L-point DFT of
+ *   x_k = in[b*n + j + k*stride] * w_N^((j % ns) * k * tw_scale),  k < L,
+ * and writes result o to out[b*n + (j/ns)*ns*L + (j%ns) + o*ns]
+ * (one radix-L Stockham step of the whole n-point transform). */
+typedef struct {
+  const void *in;
+  void *out;
+  const void *stage_tw;   /* sub-stage twiddles of the L-point kernel */
+  const void *tw_hi;      /* w_N^(h << split_shift) */
+  const void *tw_lo;      /* w_N^l, l < 2^split_shift */
+  int64_t n;
+  int64_t stride;
+  int64_t ns;
+  int64_t tw_scale;       /* n / (ns * L) */
+  int64_t columns;        /* transforms * stride */
+  uint32_t split_shift;
+  uint32_t flags;         /* MOFFT_PASS_* */
+} mofft_pass_args;
+
+#define MOFFT_PASS_SWAP_IN 1u   /* exchange re/im on load (backward) */
+#define MOFFT_PASS_SWAP_OUT 2u  /* exchange re/im on store (backward) */
+#define MOFFT_PASS_TWIDDLE 4u   /* apply the w_N pass twiddle on load */
+
+/* One compiled pass kernel (host-side copy of the generated registry). */
+typedef struct {
+  int precision_bits;
+  int length;
+  int rows;               /* 1: columns are whole contiguous transforms */
+  int threads_per_column;
+  int columns_per_block;
+  int stage_count;
+  int stages[8];          /* execution order */
+  int lds_bytes;
+} mofft_registry_entry;
+
+extern const mofft_registry_entry mofft_registry[];
+extern const int mofft_registry_count;
+
+/* Device layer.  Return 0 on success, nonzero HIP error otherwise. */
+int mofft_hip_device_available(void);
+int mofft_hip_malloc(void **ptr, size_t bytes);
+int mofft_hip_free(void *ptr);
+int mofft_hip_is_device_pointer(const void *ptr);
+int mofft_hip_copy(void *dst, const void *src, size_t bytes);
+int mofft_hip_copy_async(void *dst, const void *src, size_t bytes,
+                         void *stream);
+int mofft_hip_synchronize(void *stream);
+/* out[i] = w_M^((i * step) mod M) for i < count, rounded once from FP64. */
+int mofft_hip_fill_powers(void *out, int precision_bits, int64_t count,
+                          int64_t M, int64_t step, void *stream);
+/* Stockham sub-stage table: out[a*(r-1) + k-1] = w_(ns*r)^(a*k),
+ * a < ns, 1 <= k < r, rounded once from FP64. */
+int mofft_hip_fill_stage(void *out, int precision_bits, int64_t ns, int r,
+                         void *stream);
+int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
+                          void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

