@@ -1,0 +1,279 @@
+#!/usr/bin/env python3
+"""Benchmark of the MoFFT 1-D C2C path on one MI355X.
+
+Default workload (one "step"): 256 FP64 transforms of N = 2^16, forward then
+backward, in place, on a 256 MiB device buffer with iid N(0,1) components -
+the FP64 forward+backward in-place configuration the metric is quoted on.
+Prints ONE JSON line:
+
+  value      GFLOP/s under the 5*N*log2(N) convention (tools/benchmark.cpp:709),
+             both directions counted, whole job, inputs resident in HBM
+  roofline   dominant kernel vs the 8 TB/s HBM3E peak.  Algorithmic bytes are
+             one read + one write of every point per transform direction
+             (2*N*16 B per FP64 transform); a plan with P HBM passes is
+             charged 1/P of that per pass launch, so `frac` is the
+             end-to-end fraction of the HBM roofline.  `traffic` is the
+             measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE,
+             profiles/), or null when no counter summary is present.
+  cpu_baseline  the reference algorithm restated in C (oracle/), one core,
+             on a bounded sample of the same workload (FFTW 3.3.11, the
+             comparison the reference uses, is not available offline).
+
+Other workloads (--workload c1|c3) print their own lines for study; the
+driver's line is the default.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--sweep-max", type=int, default=20)
+    return ap.parse_args()
+
+
+def percentile(values, q):
+    v = sorted(values)
+    if not v:
+        return float("nan")
+    k = (len(v) - 1) * q
+    lo, hi = math.floor(k), math.ceil(k)
+    return v[lo] + (v[hi] - v[lo]) * (k - lo)
+
+
+def cpu_baseline_port(n, bits, seconds):
+    """Reference executor restated (oracle/), one thread, forward+backward of
+    single transforms of length n until `seconds` of CPU work are spent."""
+    import numpy as np
+    import oracle
+    from mofft_amd import describe
+    chain = describe(n, 1, bits).chain
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    count = 0
+    t0 = time.perf_counter()
+    while True:
+        y = oracle.ref_execute(x, chain, -1, bits)
+        oracle.ref_execute(y, chain, 1, bits)
+        count += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    flops = 2 * 5.0 * n * math.log2(n) * count
+    return {"value": round(flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{count} FP{bits} transforms of N={n}, forward+backward, "
+                      f"radix chain {chain}, {el:.1f} s on one host core"}
+
+
+def load_traffic(kernel_key):
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+        return data.get(kernel_key)
+    except Exception:
+        return None
+
+
+def measure_copy(torch, dev, stream, nbytes):
+    """Device-to-device copy bandwidth (read + write bytes / s) of a plain
+    copy kernel, the practical ceiling the roofline fraction is read against."""
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    return round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def run_gpu(args):
+    import torch
+    import mofft_amd as mf
+
+    if not torch.cuda.is_available():
+        raise SystemExit("bench.py needs a GPU")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261203)
+
+    if args.workload == "c2":
+        n, batch, bits = 1 << 16, 256, 64
+        fwd = mf.Plan(n, batch, 64, mf.FORWARD, in_place=True)
+        bwd = mf.Plan(n, batch, 64, mf.BACKWARD, in_place=True)
+        buf = torch.view_as_complex(torch.randn(batch, n, 2, dtype=torch.float64,
+                                                device=dev, generator=g))
+        plans = [(fwd, buf, buf), (bwd, buf, buf)]
+        directions = 2
+        workload = "fp64 c2c forward+backward in-place N=65536 batch=256"
+        dtype = "f64"
+    elif args.workload == "c1":
+        n, batch, bits = 1024, 64, 32
+        fwd = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False)
+        x = torch.rand(batch, n, 2, dtype=torch.float32, device=dev, generator=g) * 2 - 1
+        x = torch.view_as_complex(x)
+        y = torch.empty_like(x)
+        plans = [(fwd, x, y)]
+        directions = 1
+        workload = "fp32 c2c forward out-of-place N=1024 batch=64"
+        dtype = "f32"
+    else:
+        return run_sweep(args)
+
+    desc = plans[0][0].description()
+    passes = desc.pass_count
+    chunks = math.ceil(batch / desc.chunk_transforms)
+    launches_per_step = sum(p.description().pass_count * chunks for p, _, _ in plans)
+    kinfo = mf.kernels()[desc.pass_kernel[0]]
+    kernel_key = f"f{bits}_L{kinfo.length}_{'rows' if kinfo.rows else 'cols'}"
+
+    def step():
+        for p, a, b in plans:
+            p.execute_async(a, b, sh)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record(stream)
+    for i in range(args.steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    t_end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms = total_ms / args.steps
+    flops = directions * 5.0 * n * math.log2(n) * batch
+    gflops = flops / (ms * 1e-3) / 1e9
+    cb = 8 if bits == 32 else 16
+    alg_bytes_step = directions * 2.0 * n * batch * cb
+    launch_ms = total_ms / (args.steps * launches_per_step)
+    alg_per_launch = alg_bytes_step / launches_per_step
+    achieved = alg_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = load_traffic(kernel_key)
+    copy_gbs = measure_copy(torch, dev, stream, 256 << 20)
+    line = {
+        "metric": "batched C2C FFT GFLOP/s (5N log2N) + % HBM roofline; p50/p99 latency",
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "p50_ms": round(percentile(step_ms, 0.5), 4),
+        "p99_ms": round(percentile(step_ms, 0.99), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dtype,
+        "data": "synthetic (seeded on device)",
+        "config": {"workload": workload, "length": n, "batch": batch,
+                   "precision_bits": bits, "passes": passes,
+                   "pass_lengths": desc.passes, "chunk_transforms": desc.chunk_transforms,
+                   "radices": desc.chain, "parallelism": "single GPU"},
+        "hbm_gbs": round(alg_bytes_step / (ms * 1e-3) / 1e9, 1),
+        "copy_gbs": copy_gbs,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": kernel_key,
+                     "launches_per_step": launches_per_step,
+                     "avg_launch_us": round(launch_ms * 1e3, 3)},
+        "wall_s": round(wall, 3),
+    }
+    for p, _, _ in plans:
+        p.destroy()
+    return line, (n, bits)
+
+
+def run_sweep(args):
+    """FP32 forward out-of-place, N = 2^k, batch = 2^24 / N (one line per N)."""
+    import torch
+    import mofft_amd as mf
+    dev = torch.device("cuda:0")
+    sh = torch.cuda.current_stream(dev).cuda_stream
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+    total = 1 << 24
+    x = torch.view_as_complex(torch.rand(total, 2, device=dev, generator=g) * 2 - 1)
+    y = torch.empty_like(x)
+    for k in range(4, args.sweep_max + 1):
+        n = 1 << k
+        batch = total // n
+        p = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False)
+        for _ in range(3):
+            p.execute_async(x, y, sh)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record()
+        for _ in range(reps):
+            p.execute_async(x, y, sh)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        gf = 5.0 * n * k * batch / (ms * 1e-3) / 1e9
+        gbs = 2.0 * total * 8 / (ms * 1e-3) / 1e9
+        print(json.dumps({"workload": "fp32 forward out-of-place sweep", "length": n,
+                          "batch": batch, "passes": p.description().passes,
+                          "ms": round(ms, 4), "gflops": round(gf, 1),
+                          "hbm_gbs": round(gbs, 1),
+                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
+        p.destroy()
+    return None, None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
+              "batched transforms shard across GPUs with no exchange, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+    line, shape = run_gpu(args)
+    if line is None:
+        return
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_port(shape[0], shape[1], args.cpu_seconds)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

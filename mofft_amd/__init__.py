@@ -14,7 +14,9 @@ import os
 from typing import Optional, Sequence
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libmofft.so")
+# MOFFT_LIBRARY selects another build of the same library (e.g. one made
+# with `python -m mofft_amd.build --out-dir DIR --define ...`).
+LIB_PATH = os.environ.get("MOFFT_LIBRARY") or os.path.join(_HERE, "lib", "libmofft.so")
 
 # include/mofft.h:14-35
 SUCCESS = 0

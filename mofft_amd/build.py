@@ -83,12 +83,19 @@ def _compile(cmd: list[str], src: Path, obj: Path, deps_digest: str,
     return f"built {obj.name}"
 
 
-def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -> Path:
+def build(force: bool = False, jobs: int | None = None, verbose: bool = False,
+          defines: tuple[str, ...] = (), out_dir: Path | None = None) -> Path:
+    """Compile and link.  `defines` are extra -D options for the HIP sources
+    (the kernel options documented in csrc/hip/fft_pass.h); `out_dir`
+    redirects objects and the library (default: _build/ and lib/)."""
     if shutil.which(HIPCC) is None and not Path(HIPCC).exists():
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     _generate()
-    BUILD.mkdir(parents=True, exist_ok=True)
-    LIB_DIR.mkdir(parents=True, exist_ok=True)
+    build_dir = Path(out_dir) / "obj" if out_dir else BUILD
+    lib_dir = Path(out_dir) if out_dir else LIB_DIR
+    lib = lib_dir / "libmofft.so"
+    build_dir.mkdir(parents=True, exist_ok=True)
+    lib_dir.mkdir(parents=True, exist_ok=True)
     c_srcs, hip_srcs, headers = _sources()
     hdr = hashlib.sha256()
     for p in headers:
@@ -98,7 +105,7 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
     tasks = []
     objs = []
     for src in c_srcs:
-        obj = BUILD / (src.stem + src.suffix + ".o")
+        obj = build_dir / (src.stem + src.suffix + ".o")
         if src.suffix == ".cpp":
             cmd = [CXX, "-O2", "-fPIC", "-std=c++17", "-I", str(CSRC),
                    "-c", str(src), "-o", str(obj)]
@@ -108,9 +115,9 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
         tasks.append((cmd, src, obj))
         objs.append(obj)
     for src in hip_srcs:
-        obj = BUILD / (src.stem + ".hip.o")
-        cmd = [HIPCC, *HIP_FLAGS, "-I", str(GEN), "-I", str(CSRC / "hip"),
-               "-c", str(src), "-o", str(obj)]
+        obj = build_dir / (src.stem + ".hip.o")
+        cmd = [HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-I", str(GEN),
+               "-I", str(CSRC / "hip"), "-c", str(src), "-o", str(obj)]
         tasks.append((cmd, src, obj))
         objs.append(obj)
     jobs = jobs or min(16, os.cpu_count() or 4)
@@ -121,23 +128,27 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False) -
             msg = f.result()
             if verbose:
                 print(msg)
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp),
            *[str(o) for o in objs]]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")
-    os.replace(tmp, LIB)
-    (LIB_DIR / "BUILD_ID").write_text(bid + "\n")
-    return LIB
+    os.replace(tmp, lib)
+    (lib_dir / "BUILD_ID").write_text(bid + "\n")
+    return lib
 
 
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
     ap.add_argument("--jobs", type=int, default=None)
+    ap.add_argument("--define", action="append", default=[],
+                    help="extra NAME=VALUE for the HIP sources")
+    ap.add_argument("--out-dir", type=Path, default=None)
     args = ap.parse_args()
-    lib = build(force=args.force, jobs=args.jobs, verbose=True)
+    lib = build(force=args.force, jobs=args.jobs, verbose=True,
+                defines=tuple(args.define), out_dir=args.out_dir)
     print(lib)
 
 

@@ -96,7 +96,7 @@ def emit_registry_c(es: list[registry.Entry]) -> str:
     for e in es:
         st = ", ".join(str(r) for r in e.stages)
         lines.append(f"  {{{e.precision}, {e.length}, {e.rows}, {e.tpt}, {e.cols}, "
-                     f"{len(e.stages)}, {{{st}}}, {e.lds_bytes}}},")
+                     f"{len(e.stages)}, {{{st}}}, {e.lds_bytes}, {e.variant}}},")
     lines += ["};", f"const int mofft_registry_count = {len(es)};", ""]
     return "\n".join(lines)
 
@@ -153,7 +153,8 @@ def emit(out_dir: str | Path, parts: int = 8) -> dict:
         "target": "gfx950",
         "entries": [{"precision": e.precision, "length": e.length, "rows": e.rows,
                      "threads_per_column": e.tpt, "columns_per_block": e.cols,
-                     "stages": list(e.stages), "lds_bytes": e.lds_bytes}
+                     "stages": list(e.stages), "lds_bytes": e.lds_bytes,
+                     "variant": e.variant}
                     for e in es],
         "codelet_ops": {str(k): v for k, v in counts.items()},
         "digest": digest.hexdigest(),

@@ -8,7 +8,10 @@ An entry is one instantiation of ``fft_pass_kernel`` (csrc/hip/fft_pass.h):
   radix lists, include/mofft.h:138-139);
 * ``rows``: 1 when a column is a whole contiguous transform (single-pass
   plans), 0 for strided columns of a multi-pass plan;
-* ``tpt`` threads per column and ``cols`` columns per workgroup.
+* ``tpt`` threads per column and ``cols`` columns per workgroup;
+* ``variant``: the tile-shape family (the plan's kernel bank).  Bank 0 is
+  the default shape; bank 1 doubles and bank 2 halves the columns per
+  workgroup, for power-of-two lengths >= 64 where the alternative fits.
 
 The reference's radix set {64, 32, 16, 15, ..., 2} (src/mofft.c:27-28) is
 covered: every radix has a strided single-radix entry (radix 32 and 64 run as
@@ -22,7 +25,8 @@ from dataclasses import dataclass
 
 from .codelets import RADICES
 
-LDS_CAP = 96 * 1024          # bytes of dynamic LDS a registry entry may use
+LDS_CAP = 150 * 1024         # bytes of dynamic LDS a registry entry may use
+                             # (gfx950: 160 KiB per CU)
 STAGE_RADICES = (16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2)
 
 # lengths the reference's own benchmark and tests use (tools/benchmark.cpp:28-41,
@@ -40,6 +44,7 @@ class Entry:
     tpt: int
     cols: int
     stages: tuple[int, ...]
+    variant: int = 0
 
     @property
     def complex_bytes(self) -> int:
@@ -52,7 +57,7 @@ class Entry:
         row_words = self.cols * lp
         col_words = self.length * self.cols + ((self.length * self.cols) >> 5) + 1
         words = row_words if self.rows else max(row_words, col_words)
-        return 2 * words * real
+        return words * real
 
     @property
     def threads(self) -> int:
@@ -62,7 +67,8 @@ class Entry:
     def symbol(self) -> str:
         kind = "rows" if self.rows else "cols"
         chain = "x".join(str(r) for r in self.stages)
-        return f"mofft_pass_f{self.precision}_{kind}_L{self.length}_{chain}_t{self.tpt}_c{self.cols}"
+        return (f"mofft_pass_f{self.precision}_{kind}_L{self.length}_{chain}"
+                f"_t{self.tpt}_c{self.cols}_v{self.variant}")
 
 
 def chain_for(length: int, cap: int = 16) -> tuple[int, ...]:
@@ -112,7 +118,9 @@ def _pow2_chain(length: int) -> tuple[int, ...]:
 
 
 def _tpt(stages: tuple[int, ...], length: int) -> int:
-    return max(1, length // max(stages))
+    # one butterfly of the largest radix per thread, at most 1024 threads
+    # per column (larger lengths give each thread several butterflies)
+    return max(1, min(1024, length // max(stages)))
 
 
 def _make(precision: int, length: int, rows: int,
@@ -120,7 +128,8 @@ def _make(precision: int, length: int, rows: int,
     tpt = _tpt(stages, length)
     real = 4 if precision == 32 else 8
     if rows:
-        cols = max(1, 256 // tpt)
+        # 128-thread workgroups measured best for whole-transform kernels
+        cols = max(1, 128 // tpt)
     else:
         # at least 128 B of adjacent columns per access, at least 256 threads
         cols = max(16 if precision == 32 else 8, 256 // tpt)
@@ -135,6 +144,21 @@ def _make(precision: int, length: int, rows: int,
     return e
 
 
+def _variants(e: Entry) -> list[Entry]:
+    """Bank 1 (twice the columns) and bank 2 (half the columns) of a
+    power-of-two entry, where they fit the thread and LDS limits."""
+    if e.length < 64 or e.length & (e.length - 1):
+        return []
+    out = []
+    for variant, cols in ((1, e.cols * 2), (2, e.cols // 2)):
+        if cols < 1 or (not e.rows and cols < 4):
+            continue
+        v = Entry(e.precision, e.length, e.rows, e.tpt, cols, e.stages, variant)
+        if v.threads <= 1024 and v.lds_bytes <= LDS_CAP:
+            out.append(v)
+    return out
+
+
 def entries() -> list[Entry]:
     out: list[Entry] = []
     seen = set()
@@ -143,8 +167,9 @@ def entries() -> list[Entry]:
         if e is not None and (e.precision, e.length, e.rows, e.stages) not in seen:
             seen.add((e.precision, e.length, e.rows, e.stages))
             out.append(e)
+            out.extend(_variants(e))
 
-    for precision, rows_max, cols_max in ((32, 8192, 1024), (64, 4096, 1024)):
+    for precision, rows_max, cols_max in ((32, 32768, 1024), (64, 16384, 1024)):
         # single-pass kernels over whole transforms
         for k in range(1, rows_max.bit_length()):
             length = 1 << k
@@ -184,7 +209,8 @@ def describe(es: list[Entry]) -> str:
     for i, e in enumerate(es):
         lines.append(f"{i:3d} f{e.precision} {'rows' if e.rows else 'cols'} "
                      f"L={e.length:5d} stages={list(e.stages)} tpt={e.tpt} "
-                     f"cols={e.cols} threads={e.threads} lds={e.lds_bytes}")
+                     f"cols={e.cols} threads={e.threads} lds={e.lds_bytes} "
+                     f"bank={e.variant}")
     return "\n".join(lines)
 
 

@@ -62,6 +62,18 @@ __global__ void fill_stage_kernel(T *out, int64_t ns, int r) {
   out[2 * i] = (T)c;
   out[2 * i + 1] = (T)s;
 }
+template <typename T>
+__global__ void fill_pass_kernel(T *out, int64_t ns, int64_t L) {
+  const int64_t count = ns * L;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t k = i / ns;
+  const int64_t a = i - k * ns;
+  double c, s;
+  omega_f64((uint64_t)(a * k), (uint64_t)(ns * L), c, s);
+  out[2 * i] = (T)c;
+  out[2 * i + 1] = (T)s;
+}
 }  // namespace mofft
 
 extern "C" {
@@ -147,6 +159,20 @@ int mofft_hip_fill_stage(void *out, int precision_bits, int64_t ns, int r,
   else
     hipLaunchKernelGGL(mofft::fill_stage_kernel<double>, dim3(blocks),
                        dim3(256), 0, (hipStream_t)stream, (double *)out, ns, r);
+  return (int)hipGetLastError();
+}
+
+int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
+                        void *stream) {
+  const int64_t count = ns * L;
+  if (count <= 0) return 0;
+  const unsigned blocks = (unsigned)((count + 255) / 256);
+  if (precision_bits == 32)
+    hipLaunchKernelGGL(mofft::fill_pass_kernel<float>, dim3(blocks), dim3(256),
+                       0, (hipStream_t)stream, (float *)out, ns, L);
+  else
+    hipLaunchKernelGGL(mofft::fill_pass_kernel<double>, dim3(blocks),
+                       dim3(256), 0, (hipStream_t)stream, (double *)out, ns, L);
   return (int)hipGetLastError();
 }
 

@@ -20,6 +20,12 @@
  *     When the pass output interleaves fewer than C columns (ns < C), the
  *     results go through LDS and leave as one contiguous block.
  *
+ * LDS holds one real array: an exchange moves the real parts, then the
+ * imaginary parts, through the same buffer.  That halves LDS per thread and
+ * doubles the workgroups a CU can keep resident (LDS, not VGPRs, bounds
+ * occupancy for these kernels); the price is two extra barriers per
+ * exchange.
+ *
  * Reference counterpart: the per-stage kernels `mofft_kernel_r*_first/other`
  * (compiler/mofft_compiler/emitter.py:283-466) and the executor's transposes
  * (src/mofft.c:1987-2044); here several stages fuse into one HBM pass.
@@ -32,6 +38,27 @@
 #include "../mofft_abi.h"
 #include "mofft_codelets.h"
 
+/* Build-time kernel options (defaults are the measured best on MI355X):
+ *   MOFFT_NT_LOAD / MOFFT_NT_STORE  non-temporal hints on the streaming HBM
+ *                                   input / output accesses (off: they cost
+ *                                   up to 4x bandwidth on gfx950)
+ *   MOFFT_SPLIT_EXCHANGE            exchange real and imaginary parts through
+ *                                   one LDS array in turn (half the LDS)
+ *   MOFFT_VEC_SMALL                 16-byte accesses when one thread owns a
+ *                                   whole FP32 transform */
+#ifndef MOFFT_NT_LOAD
+#define MOFFT_NT_LOAD 0
+#endif
+#ifndef MOFFT_NT_STORE
+#define MOFFT_NT_STORE 0
+#endif
+#ifndef MOFFT_SPLIT_EXCHANGE
+#define MOFFT_SPLIT_EXCHANGE 1
+#endif
+#ifndef MOFFT_VEC_SMALL
+#define MOFFT_VEC_SMALL 1
+#endif
+
 namespace mofft {
 
 template <typename T> struct cx_of;
@@ -39,9 +66,8 @@ template <> struct cx_of<float> { using type = float2; };
 template <> struct cx_of<double> { using type = double2; };
 template <typename T> using cx = typename cx_of<T>::type;
 
-/* LDS tile geometry.  Real and imaginary parts live in separate arrays.
- * Column-major (ROWS) tiles pad one word every 16; interleaved (COLS) tiles
- * pad one word every 32. */
+/* LDS tile geometry (one array of T).  Column-major (ROWS) tiles pad one
+ * word every 16; interleaved (COLS) tiles pad one word every 32. */
 template <typename T, int L, int TPT, int C, bool ROWS>
 struct Tile {
   static constexpr int NT = TPT * C;
@@ -51,7 +77,8 @@ struct Tile {
   static constexpr int WORDS = ROWS ? ROW_WORDS
                                     : (ROW_WORDS > COL_WORDS ? ROW_WORDS
                                                              : COL_WORDS);
-  static constexpr int BYTES = 2 * WORDS * (int)sizeof(T);
+  static constexpr int ARRAYS = MOFFT_SPLIT_EXCHANGE ? 1 : 2;
+  static constexpr int BYTES = ARRAYS * WORDS * (int)sizeof(T);
 
   static __device__ __forceinline__ int row_addr(int c, int e) {
     return c * LP + e + (e >> 4);
@@ -68,13 +95,13 @@ struct Tile {
 
 template <typename T>
 struct Ctx {
-  T *sre;
-  T *sim;
+  T *lds;
   const cx<T> *src;   /* element e of this thread's column: src[e * stride] */
   cx<T> *dst;         /* output base of this thread's transform */
   const cx<T> *stw;
   const cx<T> *thi;
   const cx<T> *tlo;
+  const cx<T> *tpass; /* &tw_pass[j % ns] when the full table is used */
   int64_t stride;
   int64_t ns;
   int64_t obase;      /* (j / ns) * ns * L + j % ns */
@@ -87,7 +114,43 @@ struct Ctx {
   bool swap_in;
   bool swap_out;
   bool pass_tw;
+  bool pass_table;
 };
+
+/* Native clang vectors for the non-temporal load/store builtins. */
+typedef float nf2 __attribute__((ext_vector_type(2)));
+typedef float nf4 __attribute__((ext_vector_type(4)));
+typedef double nd2 __attribute__((ext_vector_type(2)));
+template <typename T> struct nvec2;
+template <> struct nvec2<float> { using type = nf2; };
+template <> struct nvec2<double> { using type = nd2; };
+
+template <typename T>
+__device__ __forceinline__ cx<T> ld_stream(const cx<T> *p) {
+  using V = typename nvec2<T>::type;
+#if MOFFT_NT_LOAD
+  const V v = __builtin_nontemporal_load(reinterpret_cast<const V *>(p));
+#else
+  const V v = *reinterpret_cast<const V *>(p);
+#endif
+  cx<T> r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void st_stream(cx<T> *p, T a, T b) {
+  using V = typename nvec2<T>::type;
+  V v;
+  v.x = a;
+  v.y = b;
+#if MOFFT_NT_STORE
+  __builtin_nontemporal_store(v, reinterpret_cast<V *>(p));
+#else
+  *reinterpret_cast<V *>(p) = v;
+#endif
+}
 
 template <typename T>
 __device__ __forceinline__ void cmul(T &re, T &im, const cx<T> w) {
@@ -98,111 +161,237 @@ __device__ __forceinline__ void cmul(T &re, T &im, const cx<T> w) {
 }
 
 template <typename T>
+__device__ __forceinline__ void pass_twiddle(const Ctx<T> &x, int e, T &a,
+                                             T &b) {
+  if (x.pass_table) {
+    /* adjacent columns read adjacent entries: coalesced */
+    cmul<T>(a, b, x.tpass[(int64_t)e * x.ns]);
+    return;
+  }
+  const uint64_t m = (uint64_t)e * x.jm_scale;
+  const cx<T> h = x.thi[m >> x.split_shift];
+  const cx<T> l = x.tlo[m & x.split_mask];
+  cx<T> w;
+  w.x = h.x * l.x - h.y * l.y;
+  w.y = h.x * l.y + h.y * l.x;
+  cmul<T>(a, b, w);
+}
+
+template <typename T>
 __device__ __forceinline__ void load_global(const Ctx<T> &x, int e, T &a,
                                             T &b) {
-  const cx<T> v = x.src[(int64_t)e * x.stride];
+  const cx<T> v = ld_stream<T>(x.src + (int64_t)e * x.stride);
   a = x.swap_in ? v.y : v.x;
   b = x.swap_in ? v.x : v.y;
-  if (x.pass_tw) {
-    const uint64_t m = (uint64_t)e * x.jm_scale;
-    const cx<T> h = x.thi[m >> x.split_shift];
-    const cx<T> l = x.tlo[m & x.split_mask];
-    cx<T> w;
-    w.x = h.x * l.x - h.y * l.y;
-    w.y = h.x * l.y + h.y * l.x;
-    cmul<T>(a, b, w);
-  }
+  if (x.pass_tw) pass_twiddle<T>(x, e, a, b);
 }
 
 template <typename T>
 __device__ __forceinline__ void store_global(cx<T> *p, T a, T b, bool swap) {
-  cx<T> v;
-  v.x = swap ? b : a;
-  v.y = swap ? a : b;
-  *p = v;
+  st_stream<T>(p, swap ? b : a, swap ? a : b);
 }
 
 /* Cooperative store of the whole tile, used when the pass output of
- * adjacent columns is not contiguous (ns < C).  The tile holds C columns of
- * L outputs in row layout. */
-template <typename T, int L, int TPT, int C, bool ROWS>
+ * adjacent columns is not contiguous (ns < C).  Each thread owns the tile
+ * elements q = tid + i*NT; real and imaginary parts pass through LDS in
+ * turn (row layout). */
+template <typename T, int L, int TPT, int C, bool ROWS, int IT, int R>
 __device__ __forceinline__ void store_tile(const Ctx<T> &x,
-                                           const mofft_pass_args &p) {
+                                           const mofft_pass_args &p,
+                                           T (&vr)[IT][R], T (&vi)[IT][R]) {
   using TL = Tile<T, L, TPT, C, ROWS>;
-  cx<T> *out = (cx<T> *)p.out;
+  constexpr int NB = L / R;
+  constexpr bool FULL = IT * TPT == NB;
+  constexpr int PER = (C * L) / TL::NT;
   const int tid = threadIdx.x;
+  const int64_t ns = p.ns;
+  const int span = (int)ns * L;
+  int ad[PER];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int q = tid + i * TL::NT;
+    int cc, o;
+    if (ns == 1) {
+      cc = q / L;
+      o = q - cc * L;
+    } else {
+      const int g = q / span;
+      const int rem = q - g * span;
+      o = rem / (int)ns;
+      cc = g * (int)ns + (rem - o * (int)ns);
+    }
+    ad[i] = TL::row_addr(cc, o);
+  }
+  T outr[PER], outi[PER];
+#pragma unroll
+  for (int half = 0; half < 2; ++half) {
+    if (half) __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int jj = x.t + it * TPT;
+      if (FULL || jj < NB) {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          x.lds[TL::row_addr(x.c, jj + k * NB)] = half ? vi[it][k] : vr[it][k];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (half) outi[i] = x.lds[ad[i]];
+      else outr[i] = x.lds[ad[i]];
+    }
+  }
+  cx<T> *out = (cx<T> *)p.out;
   const int64_t col0 = (int64_t)blockIdx.x * C;
   const int64_t b0 = col0 / p.stride;
   const int64_t j0 = col0 - b0 * p.stride;
-  const int64_t ns = p.ns;
   const bool swap = (p.flags & MOFFT_PASS_SWAP_OUT) != 0;
   const bool aligned = (j0 + C <= p.stride) && (C % ns == 0) &&
                        (j0 % ns == 0) && (col0 + C <= p.columns);
   if (aligned) {
     cx<T> *base = out + b0 * p.n + (j0 / ns) * ns * L;
-    const int span = (int)ns * L;
-#pragma unroll 4
-    for (int q = tid; q < C * L; q += TL::NT) {
-      const int g = q / span;
-      const int rem = q - g * span;
-      const int o = rem / (int)ns;
-      const int r = rem - o * (int)ns;
-      const int cc = g * (int)ns + r;
-      const int ad = TL::row_addr(cc, o);
-      store_global<T>(base + q, x.sre[ad], x.sim[ad], swap);
-    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i)
+      store_global<T>(base + tid + i * TL::NT, outr[i], outi[i], swap);
   } else {
-    for (int q = tid; q < C * L; q += TL::NT) {
-      const int cc = q / L;
-      const int o = q - cc * L;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = tid + i * TL::NT;
+      int cc, o;
+      if (ns == 1) {
+        cc = q / L;
+        o = q - cc * L;
+      } else {
+        const int g = q / span;
+        const int rem = q - g * span;
+        o = rem / (int)ns;
+        cc = g * (int)ns + (rem - o * (int)ns);
+      }
       const int64_t col = col0 + cc;
       if (col >= p.columns) continue;
       const int64_t b = col / p.stride;
       const int64_t j = col - b * p.stride;
       const int64_t pos = b * p.n + (j / ns) * ns * L + (j % ns) + (int64_t)o * ns;
-      const int ad = TL::row_addr(cc, o);
-      store_global<T>(out + pos, x.sre[ad], x.sim[ad], swap);
+      store_global<T>(out + pos, outr[i], outi[i], swap);
     }
   }
 }
 
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
-          bool FIRST, int R, int... REST>
+          int R, int... REST>
 struct Stage {
   using TL = Tile<T, L, TPT, C, ROWS>;
+  static constexpr int RAD = R;
+  static constexpr int NB = L / R;
+  static constexpr int IT = (NB + TPT - 1) / TPT;
+  static constexpr bool FULL = IT * TPT == NB;
+  static constexpr bool LAST = sizeof...(REST) == 0;
 
-  static __device__ __forceinline__ void run(const Ctx<T> &x,
-                                             const mofft_pass_args &p) {
-    constexpr int NB = L / R;
-    constexpr int IT = (NB + TPT - 1) / TPT;
-    constexpr bool FULL = IT * TPT == NB;
-    constexpr bool LAST = sizeof...(REST) == 0;
-    T vr[IT][R];
-    T vi[IT][R];
+  /* first sub-stage: element e = jj + k*NB straight from HBM */
+  static __device__ __forceinline__ void load_first(const Ctx<T> &x,
+                                                    T (&vr)[IT][R],
+                                                    T (&vi)[IT][R]) {
+    if constexpr (MOFFT_VEC_SMALL && ROWS && TPT == 1 && IT == 1 &&
+                  (R % 2 == 0) && sizeof(T) == 4) {
+      /* a whole small transform per thread: 16-byte loads */
+      const nf4 *s = reinterpret_cast<const nf4 *>(x.src);
+#pragma unroll
+      for (int k = 0; k < R; k += 2) {
+        nf4 v = {0.f, 0.f, 0.f, 0.f};
+#if MOFFT_NT_LOAD
+        if (x.live) v = __builtin_nontemporal_load(s + k / 2);
+#else
+        if (x.live) v = s[k / 2];
+#endif
+        vr[0][k] = x.swap_in ? v.y : v.x;
+        vi[0][k] = x.swap_in ? v.x : v.y;
+        vr[0][k + 1] = x.swap_in ? v.w : v.z;
+        vi[0][k + 1] = x.swap_in ? v.z : v.w;
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int jj = x.t + it * TPT;
+        const bool ok = (FULL || jj < NB) && x.live;
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          T a = T(0), b = T(0);
+          if (ok) load_global<T>(x, jj + k * NB, a, b);
+          vr[it][k] = a;
+          vi[it][k] = b;
+        }
+      }
+    }
+  }
 
+  static __device__ __forceinline__ void load_lds(const Ctx<T> &x,
+                                                  const T *arr,
+                                                  T (&v)[IT][R]) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int jj = x.t + it * TPT;
-      const bool ok = FULL || jj < NB;
 #pragma unroll
-      for (int k = 0; k < R; ++k) {
-        const int e = jj + k * NB;
-        T a = T(0), b = T(0);
-        if constexpr (FIRST) {
-          if (ok && x.live) load_global<T>(x, e, a, b);
-        } else {
-          if (ok) {
-            const int ad = TL::stage_addr(x.c, e);
-            a = x.sre[ad];
-            b = x.sim[ad];
-          }
-        }
-        vr[it][k] = a;
-        vi[it][k] = b;
+      for (int k = 0; k < R; ++k)
+        v[it][k] = (FULL || jj < NB) ? arr[TL::stage_addr(x.c, jj + k * NB)]
+                                     : T(0);
+    }
+  }
+
+  static __device__ __forceinline__ void put_lds(const Ctx<T> &x, T *arr,
+                                                 T (&v)[IT][R]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int jj = x.t + it * TPT;
+      if (FULL || jj < NB) {
+        const int base = (jj / NS) * NS * R + (jj % NS);
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          arr[TL::stage_addr(x.c, base + k * NS)] = v[it][k];
       }
     }
-    if constexpr (!FIRST) __syncthreads();
+  }
 
+  static __device__ __forceinline__ void store_direct(const Ctx<T> &x,
+                                                      T (&vr)[IT][R],
+                                                      T (&vi)[IT][R]) {
+    if (!x.live) return;
+    if constexpr (MOFFT_VEC_SMALL && ROWS && TPT == 1 && IT == 1 &&
+                  (R % 2 == 0) && sizeof(T) == 4) {
+      nf4 *d = reinterpret_cast<nf4 *>(x.dst);
+#pragma unroll
+      for (int k = 0; k < R; k += 2) {
+        nf4 v;
+        v.x = x.swap_out ? vi[0][k] : vr[0][k];
+        v.y = x.swap_out ? vr[0][k] : vi[0][k];
+        v.z = x.swap_out ? vi[0][k + 1] : vr[0][k + 1];
+        v.w = x.swap_out ? vr[0][k + 1] : vi[0][k + 1];
+#if MOFFT_NT_STORE
+        __builtin_nontemporal_store(v, d + k / 2);
+#else
+        d[k / 2] = v;
+#endif
+      }
+    } else {
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        const int jj = x.t + it * TPT;
+        if (FULL || jj < NB) {
+#pragma unroll
+          for (int k = 0; k < R; ++k) {
+            const int64_t o = jj + k * NB;
+            cx<T> *ptr = ROWS ? x.dst + o : x.dst + x.obase + o * x.ns;
+            store_global<T>(ptr, vr[it][k], vi[it][k], x.swap_out);
+          }
+        }
+      }
+    }
+  }
+
+  /* Twiddle, DFT and hand the results to the next sub-stage (or HBM). */
+  static __device__ __forceinline__ void run(const Ctx<T> &x,
+                                             const mofft_pass_args &p,
+                                             T (&vr)[IT][R], T (&vi)[IT][R]) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       if constexpr (NS > 1) {
@@ -215,73 +404,42 @@ struct Stage {
       }
       Dft<R>::template run<T>(vr[it], vi[it]);
     }
-
     if constexpr (!LAST) {
-#pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int jj = x.t + it * TPT;
-        if (FULL || jj < NB) {
-          const int base = (jj / NS) * NS * R + (jj % NS);
-#pragma unroll
-          for (int k = 0; k < R; ++k) {
-            const int ad = TL::stage_addr(x.c, base + k * NS);
-            x.sre[ad] = vr[it][k];
-            x.sim[ad] = vi[it][k];
-          }
-        }
-      }
+      using Next = Stage<T, L, TPT, C, ROWS, NS * R,
+                         OFF + (NS > 1 ? NS * (R - 1) : 0), REST...>;
+      T nr[Next::IT][Next::RAD];
+      T ni[Next::IT][Next::RAD];
+#if MOFFT_SPLIT_EXCHANGE
+      put_lds(x, x.lds, vr);
       __syncthreads();
-      Stage<T, L, TPT, C, ROWS, NS * R, OFF + (NS > 1 ? NS * (R - 1) : 0),
-            false, REST...>::run(x, p);
+      Next::load_lds(x, x.lds, nr);
+      __syncthreads();
+      put_lds(x, x.lds, vi);
+      __syncthreads();
+      Next::load_lds(x, x.lds, ni);
+      __syncthreads();
+#else
+      put_lds(x, x.lds, vr);
+      put_lds(x, x.lds + TL::WORDS, vi);
+      __syncthreads();
+      Next::load_lds(x, x.lds, nr);
+      Next::load_lds(x, x.lds + TL::WORDS, ni);
+      __syncthreads();
+#endif
+      Next::run(x, p, nr, ni);
     } else {
-      /* final sub-stage: output o = jj + k * NB of the L-point DFT */
-      const bool direct = ROWS || p.ns >= C;
-      if (direct) {
-        if (x.live) {
-#pragma unroll
-          for (int it = 0; it < IT; ++it) {
-            const int jj = x.t + it * TPT;
-            if (FULL || jj < NB) {
-#pragma unroll
-              for (int k = 0; k < R; ++k) {
-                const int64_t o = jj + k * NB;
-                cx<T> *ptr = ROWS ? x.dst + o : x.dst + x.obase + o * x.ns;
-                store_global<T>(ptr, vr[it][k], vi[it][k], x.swap_out);
-              }
-            }
-          }
-        }
-      } else {
-        if constexpr (FIRST) {
-          /* single sub-stage: LDS untouched so far */
-        }
-#pragma unroll
-        for (int it = 0; it < IT; ++it) {
-          const int jj = x.t + it * TPT;
-          if (FULL || jj < NB) {
-#pragma unroll
-            for (int k = 0; k < R; ++k) {
-              const int ad = TL::row_addr(x.c, jj + k * NB);
-              x.sre[ad] = vr[it][k];
-              x.sim[ad] = vi[it][k];
-            }
-          }
-        }
-        __syncthreads();
-        store_tile<T, L, TPT, C, ROWS>(x, p);
-      }
+      if (ROWS || p.ns >= C) store_direct(x, vr, vi);
+      else store_tile<T, L, TPT, C, ROWS, IT, R>(x, p, vr, vi);
     }
   }
 };
 
-template <typename T, int L, int TPT, int C, bool ROWS, int... R>
+template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
 __global__ void __launch_bounds__(TPT * C)
 fft_pass_kernel(mofft_pass_args p) {
-  using TL = Tile<T, L, TPT, C, ROWS>;
   extern __shared__ __align__(16) unsigned char mofft_lds[];
   Ctx<T> x;
-  x.sre = reinterpret_cast<T *>(mofft_lds);
-  x.sim = x.sre + TL::WORDS;
+  x.lds = reinterpret_cast<T *>(mofft_lds);
   const int tid = threadIdx.x;
   x.c = ROWS ? tid / TPT : tid % C;
   x.t = ROWS ? tid % TPT : tid / C;
@@ -305,12 +463,19 @@ fft_pass_kernel(mofft_pass_args p) {
   x.swap_in = (p.flags & MOFFT_PASS_SWAP_IN) != 0;
   x.swap_out = (p.flags & MOFFT_PASS_SWAP_OUT) != 0;
   x.pass_tw = (p.flags & MOFFT_PASS_TWIDDLE) != 0 && jr != 0;
-  Stage<T, L, TPT, C, ROWS, 1, 0, true, R...>::run(x, p);
+  x.pass_table = (p.flags & MOFFT_PASS_TWIDDLE_TABLE) != 0;
+  x.tpass = (const cx<T> *)p.tw_pass + jr;
+  using First = Stage<T, L, TPT, C, ROWS, 1, 0, R0, R...>;
+  T vr[First::IT][R0];
+  T vi[First::IT][R0];
+  First::load_first(x, vr, vi);
+  First::run(x, p, vr, vi);
 }
 
 template <typename T, int L, int TPT, int C, bool ROWS, int... R>
 struct PassKernel {
   static constexpr int THREADS = TPT * C;
+  static constexpr int STAGES = sizeof...(R);
   static constexpr int LDS = Tile<T, L, TPT, C, ROWS>::BYTES;
   static int launch(const mofft_pass_args *a, hipStream_t s) {
     static bool configured = false;
@@ -327,8 +492,12 @@ struct PassKernel {
     const int64_t blocks = (a->columns + C - 1) / C;
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
+    /* LDS is only touched by sub-stage exchanges and by the transposed
+     * store of strided passes whose output interleaves < C columns */
+    const bool needs_lds = STAGES > 1 || (!ROWS && a->ns < C);
     hipLaunchKernelGGL((fft_pass_kernel<T, L, TPT, C, ROWS, R...>),
-                       dim3((unsigned)blocks), dim3(THREADS), LDS, s, *a);
+                       dim3((unsigned)blocks), dim3(THREADS),
+                       needs_lds ? LDS : 0, s, *a);
     return (int)hipGetLastError();
   }
 };

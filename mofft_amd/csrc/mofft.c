@@ -43,8 +43,27 @@
 /* Device family token of recipes: the gfx target number in hex digits. */
 #define MOFFT_GPU_FAMILY 0x00000950u
 /* Multi-pass schedules stream the batch in chunks of at most this many bytes
- * so the pass-to-pass intermediate can stay in the 256 MB Infinity Cache. */
-#define MOFFT_CHUNK_BYTES ((size_t)64 << 20)
+ * (the workspace of in-place plans is one chunk).  Measured on MI355X: fewer,
+ * larger launches beat Infinity-Cache-sized chunks (FP64 N=2^16: 4.9 vs 4.0
+ * TFLOP/s at 256 vs 64 MiB) because each launch then fills all 256 CUs.
+ * MOFFT_CHUNK_BYTES in the environment overrides it (a tuning knob; read
+ * once per process). */
+#define MOFFT_DEFAULT_CHUNK_BYTES ((size_t)256 << 20)
+
+static size_t chunk_bytes(void) {
+  static size_t cached = 0;
+  if (cached == 0) {
+    size_t v = MOFFT_DEFAULT_CHUNK_BYTES;
+    const char *env = getenv("MOFFT_CHUNK_BYTES");
+    if (env && *env) {
+      char *end = NULL;
+      const unsigned long long x = strtoull(env, &end, 10);
+      if (end && *end == '\0' && x >= ((size_t)1 << 20)) v = (size_t)x;
+    }
+    cached = v;
+  }
+  return cached;
+}
 
 /* src/mofft.c:27-28 - the radix set the API accepts. */
 static const int k_radices[] = {64, 32, 16, 15, 14, 13, 12, 11, 10,
@@ -57,12 +76,22 @@ typedef struct {
   size_t span;           /* product of earlier pass lengths (ns) */
   size_t stage_offset;   /* element offset of this pass's sub-stage table */
   size_t stage_count;
+  size_t pass_offset;    /* element offset of the full pass twiddle table */
+  int pass_table;        /* 1: the full table exists for this pass */
 } mofft_pass;
+
+/* Passes after the first multiply by w_(ns*L)^((j%ns)*k).  Up to this many
+ * bytes per pass the factors come from a full table laid out so adjacent
+ * columns read adjacent entries (one coalesced load per point, L2/Infinity
+ * Cache resident); beyond it from a two-level table (two gathered loads and
+ * a complex product per point). */
+#define MOFFT_PASS_TABLE_BYTES ((size_t)8 << 20)
 
 typedef struct {
   size_t length;
   size_t batch;
   int precision_bits;
+  unsigned int bank;
   int depth;
   int factors[MOFFT_MAX_PLAN_DEPTH];   /* outer -> inner */
   uint64_t pass_mask;
@@ -123,7 +152,19 @@ static size_t complex_bytes(int precision_bits) {
   return precision_bits == 32 ? 8 : 16;
 }
 
-static unsigned int kernel_bank_count(void) { return 1; }
+/* Kernel banks are the tile-shape families of the registry (variant field):
+ * bank b runs every pass with its variant-b kernel where one is compiled,
+ * else with variant 0. */
+static unsigned int kernel_bank_count(void) {
+  static unsigned int cached = 0;
+  if (cached == 0) {
+    int v = 0;
+    for (int i = 0; i < mofft_registry_count; ++i)
+      if (mofft_registry[i].variant > v) v = mofft_registry[i].variant;
+    cached = (unsigned int)v + 1;
+  }
+  return cached;
+}
 
 int mofft_device_available(void) { return mofft_hip_device_available(); }
 
@@ -131,13 +172,13 @@ int mofft_device_available(void) { return mofft_hip_device_available(); }
 /* kernel registry lookups                                              */
 /* ------------------------------------------------------------------ */
 
-/* Registry entry whose sub-stage chain equals stages[0..count) (execution
- * order), or, for a single radix, any entry of that length. */
-static int find_kernel(int precision_bits, int rows, const int *stages,
-                       int count) {
+static int find_kernel_variant(int precision_bits, int rows, const int *stages,
+                               int count, int variant) {
   for (int i = 0; i < mofft_registry_count; ++i) {
     const mofft_registry_entry *e = &mofft_registry[i];
-    if (e->precision_bits != precision_bits || e->rows != rows) continue;
+    if (e->precision_bits != precision_bits || e->rows != rows ||
+        e->variant != variant)
+      continue;
     if (e->stage_count == count &&
         memcmp(e->stages, stages, (size_t)count * sizeof(int)) == 0)
       return i;
@@ -146,18 +187,30 @@ static int find_kernel(int precision_bits, int rows, const int *stages,
     for (int i = 0; i < mofft_registry_count; ++i) {
       const mofft_registry_entry *e = &mofft_registry[i];
       if (e->precision_bits == precision_bits && e->rows == rows &&
-          e->length == stages[0])
+          e->variant == variant && e->length == stages[0])
         return i;
     }
   }
   return -1;
 }
 
+/* Registry entry whose sub-stage chain equals stages[0..count) (execution
+ * order), or, for a single radix, any entry of that length; the bank's
+ * variant when compiled, else variant 0. */
+static int find_kernel(int precision_bits, int rows, const int *stages,
+                       int count, unsigned int bank) {
+  int k = -1;
+  if (bank != 0)
+    k = find_kernel_variant(precision_bits, rows, stages, count, (int)bank);
+  if (k < 0) k = find_kernel_variant(precision_bits, rows, stages, count, 0);
+  return k;
+}
+
 static int find_kernel_by_length(int precision_bits, int rows, size_t length) {
   for (int i = 0; i < mofft_registry_count; ++i) {
     const mofft_registry_entry *e = &mofft_registry[i];
     if (e->precision_bits == precision_bits && e->rows == rows &&
-        (size_t)e->length == length)
+        e->variant == 0 && (size_t)e->length == length)
       return i;
   }
   return -1;
@@ -203,17 +256,19 @@ static mofft_status validate_chain(size_t length, const int *radices,
 static mofft_status build_schedule(size_t length, size_t batch,
                                    int precision_bits, const int *radices,
                                    size_t depth, uint64_t mask,
-                                   mofft_schedule *s) {
+                                   unsigned int bank, mofft_schedule *s) {
   mofft_status st = validate_chain(length, radices, depth);
   if (st != MOFFT_SUCCESS) return st;
   if (batch == 0 || (precision_bits != 32 && precision_bits != 64))
     return MOFFT_INVALID_ARGUMENT;
   if (depth < 64 && depth > 0 && (mask >> (depth - 1)) != 0)
     return MOFFT_INVALID_ARGUMENT;
+  if (bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
   memset(s, 0, sizeof(*s));
   s->length = length;
   s->batch = batch;
   s->precision_bits = precision_bits;
+  s->bank = bank;
   s->depth = (int)depth;
   memcpy(s->factors, radices, depth * sizeof(int));
   int exec[MOFFT_MAX_PLAN_DEPTH];
@@ -232,7 +287,7 @@ static mofft_status build_schedule(size_t length, size_t batch,
     for (size_t len = limit; len >= 1; --len) {
       const int rows = (start == 0 && len == depth) ? 1 : 0;
       if (len > 8) continue;
-      kernel = find_kernel(precision_bits, rows, exec + start, (int)len);
+      kernel = find_kernel(precision_bits, rows, exec + start, (int)len, bank);
       if (kernel >= 0) { take = len; break; }
     }
     if (kernel < 0) return MOFFT_UNSUPPORTED_SIZE;
@@ -262,6 +317,16 @@ static mofft_status build_schedule(size_t length, size_t batch,
     s->tw_lo_offset = s->tw_hi_count;
     count = s->tw_hi_count + s->tw_lo_count;
   }
+  const size_t cb0 = complex_bytes(precision_bits);
+  for (int i = 1; i < s->pass_count; ++i) {
+    mofft_pass *p = &s->passes[i];
+    const size_t entries = p->span * p->length;
+    if (sat_mul(entries, cb0) <= MOFFT_PASS_TABLE_BYTES) {
+      p->pass_table = 1;
+      p->pass_offset = count;
+      count += entries;
+    }
+  }
   for (int i = 0; i < s->pass_count; ++i) {
     mofft_pass *p = &s->passes[i];
     const mofft_registry_entry *e = &mofft_registry[p->kernel];
@@ -277,8 +342,8 @@ static mofft_status build_schedule(size_t length, size_t batch,
   s->twiddle_count = count;
   const size_t cb = complex_bytes(precision_bits);
   const size_t transform_bytes = sat_mul(length, cb);
-  size_t chunk = transform_bytes >= MOFFT_CHUNK_BYTES
-                     ? 1 : MOFFT_CHUNK_BYTES / transform_bytes;
+  const size_t cbytes = chunk_bytes();
+  size_t chunk = transform_bytes >= cbytes ? 1 : cbytes / transform_bytes;
   if (chunk > batch) chunk = batch;
   if (chunk == 0) chunk = 1;
   s->chunk = s->pass_count > 1 ? chunk : batch;
@@ -328,7 +393,8 @@ static void search_splits(int precision_bits, size_t remaining,
   if (best->count != 0 && cur->count + 1 > best->count) return;
   for (int i = 0; i < mofft_registry_count; ++i) {
     const mofft_registry_entry *e = &mofft_registry[i];
-    if (e->precision_bits != precision_bits || e->rows) continue;
+    if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
+      continue;
     const size_t l = (size_t)e->length;
     if (l < 2 || remaining % l) continue;
     int dup = 0;
@@ -480,6 +546,12 @@ static mofft_status fill_twiddles(mofft_plan_common *p) {
   }
   for (int i = 0; i < s->pass_count && rc == 0; ++i) {
     const mofft_pass *ps = &s->passes[i];
+    if (ps->pass_table)
+      rc |= mofft_hip_fill_pass(base + ps->pass_offset * cb, s->precision_bits,
+                                (int64_t)ps->span, (int64_t)ps->length, NULL);
+  }
+  for (int i = 0; i < s->pass_count && rc == 0; ++i) {
+    const mofft_pass *ps = &s->passes[i];
     const mofft_registry_entry *e = &mofft_registry[ps->kernel];
     size_t ns = 1, off = ps->stage_offset;
     for (int k = 0; k < e->stage_count; ++k) {
@@ -501,19 +573,19 @@ static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
                                     size_t batch, int precision_bits,
                                     int direction, mofft_placement placement,
                                     const int *radices, size_t depth,
-                                    uint64_t mask) {
+                                    uint64_t mask, unsigned int bank) {
   *out = NULL;
   if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
   mofft_plan_common *p = (mofft_plan_common *)calloc(1, sizeof(*p));
   if (!p) return MOFFT_ALLOCATION_FAILURE;
   mofft_status st = build_schedule(length, batch, precision_bits, radices,
-                                   depth, mask, &p->s);
+                                   depth, mask, bank, &p->s);
   if (st != MOFFT_SUCCESS) { free(p); return st; }
   p->direction = direction;
   p->placement = placement;
   p->direct_input = 0;
   p->stage_layout = MOFFT_STAGE_LAYOUT_TRANSPOSE;
-  p->kernel_bank = 0;
+  p->kernel_bank = bank;
   st = fill_twiddles(p);
   if (st == MOFFT_SUCCESS) {
     const size_t ws = schedule_workspace(&p->s, placement);
@@ -559,6 +631,7 @@ static int launch_schedule(const mofft_plan_common *p, const void *in,
       a.stage_tw = tw ? tw + ps->stage_offset * cb : NULL;
       a.tw_hi = tw ? tw + s->tw_hi_offset * cb : NULL;
       a.tw_lo = tw ? tw + s->tw_lo_offset * cb : NULL;
+      a.tw_pass = (tw && ps->pass_table) ? tw + ps->pass_offset * cb : NULL;
       a.n = (int64_t)s->length;
       a.stride = (int64_t)ps->stride;
       a.ns = (int64_t)ps->span;
@@ -569,6 +642,7 @@ static int launch_schedule(const mofft_plan_common *p, const void *in,
       if (backward && i == 0) a.flags |= MOFFT_PASS_SWAP_IN;
       if (backward && i == P - 1) a.flags |= MOFFT_PASS_SWAP_OUT;
       if (ps->span > 1) a.flags |= MOFFT_PASS_TWIDDLE;
+      if (ps->span > 1 && ps->pass_table) a.flags |= MOFFT_PASS_TWIDDLE_TABLE;
       const int rc = mofft_hip_launch_pass(ps->kernel, &a, stream);
       if (rc != 0) return rc;
       src = dst;
@@ -654,6 +728,7 @@ typedef struct {
   int radices[MOFFT_MAX_PLAN_DEPTH];
   size_t depth;
   uint64_t mask;
+  unsigned int bank;
   double model;
   double samples[MOFFT_MAX_TIMING_SAMPLES];
   double median;
@@ -681,27 +756,42 @@ static double schedule_model_cost(const mofft_schedule *s) {
 
 static int same_candidate(const measured_candidate *a,
                           const measured_candidate *b) {
-  return a->depth == b->depth && a->mask == b->mask &&
+  return a->depth == b->depth && a->mask == b->mask && a->bank == b->bank &&
          memcmp(a->radices, b->radices, a->depth * sizeof(int)) == 0;
 }
 
+/* Adds the chain once per kernel bank whose schedule differs from bank 0's
+ * (a bank with no variant kernel for any pass is the same plan). */
 static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
                           size_t length, size_t batch, int precision_bits,
                           const int *radices, size_t depth, uint64_t mask) {
-  if (*count >= cap) return;
-  mofft_schedule s;
-  if (build_schedule(length, batch, precision_bits, radices, depth, mask, &s) !=
-      MOFFT_SUCCESS)
-    return;
-  measured_candidate c;
-  memset(&c, 0, sizeof(c));
-  memcpy(c.radices, radices, depth * sizeof(int));
-  c.depth = depth;
-  c.mask = s.pass_mask; /* effective grouping */
-  c.model = schedule_model_cost(&s);
-  for (size_t i = 0; i < *count; ++i)
-    if (same_candidate(&pool[i], &c)) return;
-  pool[(*count)++] = c;
+  mofft_schedule base;
+  for (unsigned int bank = 0; bank < kernel_bank_count(); ++bank) {
+    if (*count >= cap) return;
+    mofft_schedule s;
+    if (build_schedule(length, batch, precision_bits, radices, depth, mask,
+                       bank, &s) != MOFFT_SUCCESS)
+      return;
+    if (bank == 0) {
+      base = s;
+    } else {
+      int same = s.pass_count == base.pass_count;
+      for (int i = 0; same && i < s.pass_count; ++i)
+        same = s.passes[i].kernel == base.passes[i].kernel;
+      if (same) continue;
+    }
+    measured_candidate c;
+    memset(&c, 0, sizeof(c));
+    memcpy(c.radices, radices, depth * sizeof(int));
+    c.depth = depth;
+    c.mask = s.pass_mask; /* effective grouping */
+    c.bank = bank;
+    c.model = schedule_model_cost(&s) * (1.0 + 0.01 * bank);
+    int dup = 0;
+    for (size_t i = 0; i < *count; ++i)
+      if (same_candidate(&pool[i], &c)) { dup = 1; break; }
+    if (!dup) pool[(*count)++] = c;
+  }
 }
 
 static void enumerate_splits(int precision_bits, size_t remaining,
@@ -726,7 +816,8 @@ static void enumerate_splits(int precision_bits, size_t remaining,
   if (cur->count >= max_passes) return;
   for (int i = 0; i < mofft_registry_count; ++i) {
     const mofft_registry_entry *e = &mofft_registry[i];
-    if (e->precision_bits != precision_bits || e->rows) continue;
+    if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
+      continue;
     const size_t l = (size_t)e->length;
     if (l < 2 || remaining % l) continue;
     cur->lengths[cur->count] = l;
@@ -747,6 +838,7 @@ static int candidate_model_compare(const void *l, const void *r) {
   for (size_t i = 0; i < a->depth; ++i)
     if (a->radices[i] != b->radices[i]) return a->radices[i] < b->radices[i] ? -1 : 1;
   if (a->mask != b->mask) return a->mask < b->mask ? -1 : 1;
+  if (a->bank != b->bank) return a->bank < b->bank ? -1 : 1;
   return 0;
 }
 
@@ -828,7 +920,7 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
       measured_candidate *c = &pool[i];
       mofft_plan_common *p = NULL;
       st = plan_from_chain(&p, length, batch, precision_bits, direction,
-                           placement, c->radices, c->depth, c->mask);
+                           placement, c->radices, c->depth, c->mask, c->bank);
       if (st != MOFFT_SUCCESS) goto done;
       const void *xin = placement == MOFFT_IN_PLACE ? dout : din;
       if (placement == MOFFT_IN_PLACE) mofft_hip_copy(dout, dref, bytes);
@@ -859,7 +951,8 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
   }
   qsort(pool, selected, sizeof(pool[0]), candidate_time_compare);
   st = plan_from_chain(out, length, batch, precision_bits, direction, placement,
-                       pool[0].radices, pool[0].depth, pool[0].mask);
+                       pool[0].radices, pool[0].depth, pool[0].mask,
+                       pool[0].bank);
   if (st == MOFFT_SUCCESS && diagnostics) {
     diagnostics->measured = 1;
     diagnostics->candidates_measured = selected;
@@ -918,12 +1011,11 @@ static mofft_status create_with_options(mofft_plan_common **out,
     if (st != MOFFT_SUCCESS) return st;
   }
   st = plan_from_chain(out, length, batch, precision_bits, direction,
-                       placement, radices, depth, mask);
+                       placement, radices, depth, mask, w > 0 ? bank : 0);
   if (st != MOFFT_SUCCESS) return st;
   if (w > 0) {
     (*out)->direct_input = direct;
     (*out)->stage_layout = (mofft_stage_layout)layout;
-    (*out)->kernel_bank = bank;
   }
   if (diagnostics) {
     diagnostics->used_wisdom = w > 0;
@@ -947,7 +1039,7 @@ static mofft_status create_with_radices(mofft_plan_common **out,
   mofft_status st = validate_chain(length, radices, count);
   if (st != MOFFT_SUCCESS) return st;
   return plan_from_chain(out, length, batch, precision_bits, direction,
-                         placement, radices, count, 0);
+                         placement, radices, count, 0, 0);
 }
 
 mofft_status mofft_plan_create_f32(mofft_plan_f32 **plan, size_t length,
@@ -1221,7 +1313,7 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
   }
   mofft_schedule s;
   mofft_status st = build_schedule(length, batch, precision_bits, chain, depth,
-                                   mask, &s);
+                                   mask, 0, &s);
   if (st != MOFFT_SUCCESS) return st;
   describe_schedule(&s, MOFFT_IN_PLACE, description);
   return MOFFT_SUCCESS;
@@ -1246,13 +1338,22 @@ mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
 /* ------------------------------------------------------------------ */
 
 /* Changing the pass grouping rebuilds the schedule and twiddle tables. */
-static mofft_status regroup(mofft_plan_common *p, uint64_t mask) {
+static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
+                            unsigned int bank) {
   mofft_schedule s;
   mofft_status st = build_schedule(p->s.length, p->s.batch,
                                    p->s.precision_bits, p->s.factors,
-                                   (size_t)p->s.depth, mask, &s);
+                                   (size_t)p->s.depth, mask, bank, &s);
   if (st != MOFFT_SUCCESS) return st;
-  if (s.pass_mask == p->s.pass_mask) return MOFFT_SUCCESS;
+  int same = s.pass_count == p->s.pass_count;
+  for (int i = 0; same && i < s.pass_count; ++i)
+    same = s.passes[i].kernel == p->s.passes[i].kernel;
+  if (same) {
+    p->s.pass_mask = s.pass_mask;
+    p->s.bank = bank;
+    p->kernel_bank = bank;
+    return MOFFT_SUCCESS;
+  }
   mofft_plan_common tmp = *p;
   tmp.s = s;
   tmp.twiddles = NULL;
@@ -1273,6 +1374,7 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask) {
   p->s = s;
   p->twiddles = tmp.twiddles;
   p->workspace = tmp.workspace;
+  p->kernel_bank = bank;
   return MOFFT_SUCCESS;
 }
 
@@ -1280,7 +1382,7 @@ static mofft_status set_mask(mofft_plan_common *p, uint64_t mask) {
   if (!p) return MOFFT_INVALID_ARGUMENT;
   const int transpose_count = p->s.depth > 1 ? p->s.depth - 1 : 0;
   if (transpose_count < 64 && (mask >> transpose_count)) return MOFFT_INVALID_ARGUMENT;
-  return regroup(p, mask);
+  return regroup(p, mask, p->kernel_bank);
 }
 
 static mofft_status set_strategy(mofft_plan_common *p,
@@ -1290,7 +1392,8 @@ static mofft_status set_strategy(mofft_plan_common *p,
     return MOFFT_INVALID_ARGUMENT;
   const uint64_t all = p->s.depth <= 1 ? 0
                                        : ((uint64_t)1 << (p->s.depth - 1)) - 1;
-  return regroup(p, strategy == MOFFT_TRANSPOSE_BLOCKED ? all : 0);
+  return regroup(p, strategy == MOFFT_TRANSPOSE_BLOCKED ? all : 0,
+                 p->kernel_bank);
 }
 
 mofft_status mofft_plan_set_transpose_strategy_f32(
@@ -1346,18 +1449,18 @@ mofft_status mofft_plan_set_stage_layout_f64(mofft_plan_f64 *plan,
   return MOFFT_SUCCESS;
 }
 
+/* A kernel bank is a tile-shape family (see kernel_bank_count); changing
+ * it rebuilds the schedule with the bank's kernels. */
 mofft_status mofft_plan_set_kernel_bank_f32(mofft_plan_f32 *plan,
                                             unsigned int bank) {
   if (!plan || bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
-  plan->c.kernel_bank = bank;
-  return MOFFT_SUCCESS;
+  return regroup(&plan->c, plan->c.s.pass_mask, bank);
 }
 
 mofft_status mofft_plan_set_kernel_bank_f64(mofft_plan_f64 *plan,
                                             unsigned int bank) {
   if (!plan || bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
-  plan->c.kernel_bank = bank;
-  return MOFFT_SUCCESS;
+  return regroup(&plan->c, plan->c.s.pass_mask, bank);
 }
 
 mofft_status mofft_plan_get_kernel_bank_f32(const mofft_plan_f32 *plan,
@@ -1542,11 +1645,10 @@ static mofft_status import_plan(mofft_plan_common **out, const char *recipe,
   if (validate_chain(f.length, f.radices, f.depth) != MOFFT_SUCCESS)
     return MOFFT_INVALID_ARGUMENT;
   st = plan_from_chain(out, f.length, 1, precision_bits, f.direction,
-                       f.placement, f.radices, f.depth, f.mask);
+                       f.placement, f.radices, f.depth, f.mask, f.bank);
   if (st != MOFFT_SUCCESS) { *out = NULL; return st; }
   (*out)->direct_input = f.direct_input;
   (*out)->stage_layout = f.layout;
-  (*out)->kernel_bank = f.bank;
   if (diagnostics) {
     diagnostics->used_wisdom = 1;
     fill_diagnostics(diagnostics, *out);

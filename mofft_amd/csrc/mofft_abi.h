@@ -30,6 +30,9 @@ typedef struct {
   const void *stage_tw;   /* sub-stage twiddles of the L-point kernel */
   const void *tw_hi;      /* w_N^(h << split_shift) */
   const void *tw_lo;      /* w_N^l, l < 2^split_shift */
+  const void *tw_pass;    /* full pass table: [k * ns + (j % ns)] =
+                             w_(ns*L)^((j % ns) * k), used with
+                             MOFFT_PASS_TWIDDLE_TABLE */
   int64_t n;
   int64_t stride;
   int64_t ns;
@@ -42,6 +45,8 @@ typedef struct {
 #define MOFFT_PASS_SWAP_IN 1u   /* exchange re/im on load (backward) */
 #define MOFFT_PASS_SWAP_OUT 2u  /* exchange re/im on store (backward) */
 #define MOFFT_PASS_TWIDDLE 4u   /* apply the w_N pass twiddle on load */
+#define MOFFT_PASS_TWIDDLE_TABLE 8u  /* ... read from the coalesced full
+                                        table instead of two-level lookups */
 
 /* One compiled pass kernel (host-side copy of the generated registry). */
 typedef struct {
@@ -53,6 +58,8 @@ typedef struct {
   int stage_count;
   int stages[8];          /* execution order */
   int lds_bytes;
+  int variant;            /* tile-shape family; plans select it as their
+                             kernel bank */
 } mofft_registry_entry;
 
 extern const mofft_registry_entry mofft_registry[];
@@ -74,6 +81,10 @@ int mofft_hip_fill_powers(void *out, int precision_bits, int64_t count,
  * a < ns, 1 <= k < r, rounded once from FP64. */
 int mofft_hip_fill_stage(void *out, int precision_bits, int64_t ns, int r,
                          void *stream);
+/* Pass table: out[k*ns + a] = w_(ns*L)^(a*k), a < ns, k < L, rounded once
+ * from FP64. */
+int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
+                        void *stream);
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream);
 

@@ -105,7 +105,8 @@ def test_no_device_means_unsupported_hardware():
 
 @pytest.mark.parametrize("n,bits,passes", [
     (1024, 32, [1024]), (8192, 32, [8192]), (4096, 64, [4096]),
-    (1 << 16, 64, [256, 256]), (1 << 20, 32, [1024, 1024]), (1 << 14, 32, [128, 128]),
+    (1 << 16, 64, [256, 256]), (1 << 20, 32, [1024, 1024]), (1 << 14, 32, [16384]), (1 << 15, 32, [32768]),
+    (1 << 14, 64, [16384]), (1 << 15, 64, [256, 128]),
     (1 << 24, 32, [256, 256, 256]), (1 << 24, 64, [256, 256, 256]),
     (45, 32, [45]), (20736, 64, [144, 144]), (371293, 32, [2197, 169]),
 ])
@@ -146,7 +147,7 @@ def test_workspace_rules():
     # out-of-place two-pass plans need no workspace; in-place ones do
     d = mf.describe(1 << 20, 16, 32)
     assert d.pass_count == 2 and d.workspace_bytes > 0      # describe() reports in-place
-    assert d.chunk_transforms * (1 << 20) * 8 <= 64 << 20
+    assert d.chunk_transforms * (1 << 20) * 8 <= 256 << 20
 
 
 def test_cost_model_byte_kats():
