@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweep-max", type=int, default=20)
@@ -88,10 +88,12 @@ def load_traffic(kernel_key):
     if not os.path.exists(path):
         return None
     try:
-        data = json.load(open(path))
-        return data.get(kernel_key)
+        entry = json.load(open(path)).get(kernel_key)
     except Exception:
         return None
+    if isinstance(entry, dict):
+        return entry.get("bytes_per_launch")
+    return entry
 
 
 def measure_copy(torch, dev, stream, nbytes):
@@ -144,6 +146,13 @@ def run_gpu(args):
         directions = 1
         workload = "fp32 c2c forward out-of-place N=1024 batch=64"
         dtype = "f32"
+    elif args.workload == "copy":
+        # calibration for the HBM byte counters: one 256 MiB device copy
+        nbytes = 256 << 20
+        gbs = measure_copy(torch, dev, stream, nbytes)
+        print(json.dumps({"workload": "copy calibration", "bytes_read": nbytes,
+                          "bytes_written": nbytes, "copy_gbs": gbs}), flush=True)
+        return None, None
     else:
         return run_sweep(args)
 
