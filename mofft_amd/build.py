@@ -139,6 +139,25 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False,
     return lib
 
 
+def build_c_tests(lib: Path = LIB) -> list[Path]:
+    """Compile the drop-in C tests (tests/c/*.c) against include/ and the
+    built library; binaries go to tests/c/build/."""
+    out = []
+    src_dir = ROOT / "tests" / "c"
+    bin_dir = src_dir / "build"
+    bin_dir.mkdir(parents=True, exist_ok=True)
+    for src in sorted(src_dir.glob("*.c")):
+        exe = bin_dir / src.stem
+        cmd = [CC, "-O2", "-std=gnu11", "-Wall", "-I", str(ROOT / "include"),
+               str(src), "-o", str(exe), "-L", str(lib.parent), "-lmofft",
+               f"-Wl,-rpath,{lib.parent}", "-lm"]
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"C test build failed: {' '.join(cmd)}\n{res.stderr}")
+        out.append(exe)
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
@@ -149,6 +168,8 @@ def main() -> None:
     args = ap.parse_args()
     lib = build(force=args.force, jobs=args.jobs, verbose=True,
                 defines=tuple(args.define), out_dir=args.out_dir)
+    if args.out_dir is None:
+        build_c_tests(lib)
     print(lib)
 
 

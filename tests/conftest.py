@@ -33,8 +33,19 @@ def seeded_complex(shape, seed, dist="uniform", dtype=np.complex128):
     return (re + 1j * im).astype(dtype)
 
 
+def _torch_first():
+    """PyTorch-ROCm ships its own HIP runtime.  When a process uses both, torch
+    must load first so libmofft binds to the already-loaded runtime (same
+    soname); loading libmofft first leaves torch without a device."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 @pytest.fixture(scope="session")
 def mofft():
+    _torch_first()
     import mofft_amd
     mofft_amd.lib()
     return mofft_amd

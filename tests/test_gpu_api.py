@@ -1,0 +1,193 @@
+"""GPU tests of the API surface around the transform: device pointers and
+streams (mofft_hip.h), caller workspace, pass regrouping and kernel banks,
+recipes, MEASURE and wisdom-strict planning, twiddle tables.
+
+Every numeric check compares against the FP64 oracle with the path's bars
+(FP32 1e-6*log2N, FP64 1e-15*log2N relative L2) or asserts bit identity where
+only data movement differs (reference tests/test_fft.c:253-306).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import seeded_complex
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n, bits):
+    return (1e-6 if bits == 32 else 1e-15) * max(1.0, math.log2(n))
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available()
+    return torch
+
+
+def to_dev(torch, x, bits):
+    t = torch.from_numpy(np.ascontiguousarray(x.astype(np.complex64 if bits == 32 else np.complex128)))
+    return t.to("cuda")
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n,batch", [(1024, 64), (1 << 16, 8), (4096, 3)])
+def test_async_device_pointers(gpu, torch_dev, n, batch, bits):
+    torch = torch_dev
+    x = seeded_complex((batch, n), seed=n + batch)
+    d_in = to_dev(torch, x, bits)
+    d_out = torch.empty_like(d_in)
+    stream = torch.cuda.Stream()
+    with gpu.Plan(n, batch, bits, gpu.FORWARD) as p:
+        with torch.cuda.stream(stream):
+            p.execute_async(d_in, d_out, stream.cuda_stream)
+        stream.synchronize()
+        y = d_out.cpu().numpy()
+    ref = oracle.fft_dd(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
+    assert oracle.rel_l2(y, ref) <= tol(n, bits)
+
+
+def test_async_rejects_host_pointers(gpu):
+    x = np.zeros(64, np.complex64)
+    with gpu.Plan(64) as p:
+        with pytest.raises(gpu.MofftError) as e:
+            p.execute_async(x, x.copy())
+        assert e.value.status == gpu.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_caller_workspace_concurrent_streams(gpu, torch_dev, bits):
+    """One in-place multi-pass plan, two streams, two workspaces."""
+    torch = torch_dev
+    n, batch = 1 << 16, 4
+    xs = [seeded_complex((batch, n), seed=s) for s in (1, 2)]
+    bufs = [to_dev(torch, x, bits) for x in xs]
+    with gpu.Plan(n, batch, bits, gpu.FORWARD, in_place=True) as p:
+        need = p.workspace_size()
+        assert need > 0
+        works = [torch.empty(need, dtype=torch.uint8, device="cuda") for _ in bufs]
+        streams = [torch.cuda.Stream() for _ in bufs]
+        for b, w, s in zip(bufs, works, streams):
+            p.execute_async_workspace(b, b, w, need, s.cuda_stream)
+        for s in streams:
+            s.synchronize()
+        with pytest.raises(gpu.MofftError):
+            p.execute_async_workspace(bufs[0], bufs[0], works[0], need - 1, 0)
+    for x, b in zip(xs, bufs):
+        ref = oracle.fft_dd(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
+        assert oracle.rel_l2(b.cpu().numpy(), ref) <= tol(n, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_pass_grouping_agrees(gpu, bits):
+    """Same chain, different HBM pass boundaries (a different four-step
+    factorisation of the same stages): equal to rounding and each within the
+    oracle bar; the all-fused and all-split extremes are both checked."""
+    n = 1 << 12
+    x = seeded_complex((2, n), seed=9).astype(np.complex64 if bits == 32 else np.complex128)
+    outs = []
+    for mask in (0, 1, 2, 3):
+        with gpu.Plan(n, 2, bits, radices=[16, 16, 16], pass_mask=mask) as p:
+            y = np.empty_like(x)
+            p.execute(x, y)
+            outs.append((p.description().passes, y))
+    assert [o[0] for o in outs] == [[4096], [256, 16], [16, 256], [16, 16, 16]]
+    ref = oracle.dft(x, -1)
+    for _, y in outs:
+        assert oracle.rel_l2(y, ref) <= tol(n, bits)
+        assert oracle.rel_l2(y, outs[0][1]) <= tol(n, bits)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", [1024, 1 << 16])
+def test_every_kernel_bank(gpu, n, bits):
+    x = seeded_complex((4, n), seed=n).astype(np.complex64 if bits == 32 else np.complex128)
+    ref = oracle.fft_dd(x, -1)
+    with gpu.Plan(n, 4, bits) as p0:
+        count = p0.diagnostics.kernel_bank_count
+    assert count >= 2
+    L = gpu.lib()
+    for bank in range(count):
+        with gpu.Plan(n, 4, bits) as p:
+            assert getattr(L, f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, bank) == 0
+            got = ctypes.c_uint()
+            getattr(L, f"mofft_plan_get_kernel_bank_f{bits}")(p.handle, ctypes.byref(got))
+            assert got.value == bank
+            y = np.empty_like(x)
+            p.execute(x, y)
+        assert oracle.rel_l2(y, ref) <= tol(n, bits)
+    with gpu.Plan(n, 4, bits) as p:
+        assert getattr(L, f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, count) == gpu.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_recipe_roundtrip_replays_bits(gpu, bits):
+    n = 20736
+    x = seeded_complex((1, n), seed=3).astype(np.complex64 if bits == 32 else np.complex128)
+    with gpu.Plan(n, 1, bits, gpu.BACKWARD) as p:
+        recipe = p.export()
+        y0 = np.empty_like(x)
+        p.execute(x, y0)
+    assert recipe.startswith("MOFFT_PLAN_V1 " + gpu.build_id() + " 00000950 f%d 20736 1 0 " % bits)
+    L = gpu.lib()
+    h = ctypes.c_void_p()
+    d = gpu.PlanDiagnostics()
+    raw = recipe.encode()
+    st = getattr(L, f"mofft_plan_import_f{bits}")(ctypes.byref(h), raw, len(raw) + 1, ctypes.byref(d))
+    assert st == gpu.SUCCESS and d.length == n and d.direction == gpu.BACKWARD
+    y1 = np.empty_like(x)
+    assert getattr(L, f"mofft_execute_f{bits}")(h, x.ctypes.data, y1.ctypes.data) == 0
+    getattr(L, f"mofft_plan_destroy_f{bits}")(h)
+    assert np.array_equal(y0, y1)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n,batch,in_place", [(30, 1, False), (4096, 16, True), (1 << 16, 4, False)])
+def test_measure_planner(gpu, n, batch, in_place, bits):
+    p = gpu.Plan(n, batch, bits, gpu.FORWARD, in_place=in_place, mode=gpu.MEASURE,
+                 candidate_limit=6, timing_samples=3)
+    try:
+        d = p.diagnostics
+        assert d.measured == 1 and 1 <= d.candidates_measured <= 6
+        assert d.selected_median_seconds > 0 and d.planning_seconds > 0
+        assert math.prod(d.chain) == n
+        x = seeded_complex((batch, n), seed=n).astype(np.complex64 if bits == 32 else np.complex128)
+        if in_place:
+            y = x.copy()
+            p.execute(y, y)
+        else:
+            y = np.empty_like(x)
+            p.execute(x, y)
+    finally:
+        p.destroy()
+    assert oracle.rel_l2(y, oracle.fft_dd(x, -1)) <= tol(n, bits)
+
+
+def test_strict_wisdom_and_model_fallback(gpu):
+    with pytest.raises(gpu.MofftError) as e:
+        gpu.Plan(1024, 1, 32, flags=gpu.STRICT_WISDOM)
+    assert e.value.status == gpu.WISDOM_NOT_FOUND
+    with gpu.Plan(1024) as p:
+        assert p.diagnostics.used_model_fallback == 1 and p.diagnostics.used_wisdom == 0
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_twiddle_tables_rounded_once(gpu, torch_dev, bits):
+    """The GPU-generated tables equal the correctly rounded values (FP32
+    exactly; FP64 within one ulp - the device sincospi is not correctly
+    rounded).  Read back through a two-pass plan's impulse response."""
+    n = 1 << 16
+    # X[k] = w_N^(k*n0) for an impulse at n0: exercises stage and pass tables
+    x = np.zeros((1, n), dtype=np.complex64 if bits == 32 else np.complex128)
+    x[0, 1] = 1.0
+    with gpu.Plan(n, 1, bits) as p:
+        y = np.empty_like(x)
+        p.execute(x, y)
+    k = np.arange(0, n, 997)
+    want = oracle.omega_table(n, -1, 64, indices=k)
+    err = np.max(np.abs(y[0, k] - want))
+    assert err <= (8 * np.finfo(np.float32).eps if bits == 32 else 8 * np.finfo(np.float64).eps)
