@@ -158,6 +158,21 @@ def build_c_tests(lib: Path = LIB) -> list[Path]:
     return out
 
 
+def build_tools(lib: Path = LIB) -> list[Path]:
+    """Compile the autotuner (tune/mofft_tune.cpp) against the library."""
+    out_dir = PKG / "tune" / "build"
+    out_dir.mkdir(parents=True, exist_ok=True)
+    exe = out_dir / "mofft_tune"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-Wno-unused-command-line-argument",
+           "-I", str(ROOT / "include"), str(PKG / "tune" / "mofft_tune.cpp"),
+           "-o", str(exe), "-L", str(lib.parent), "-lmofft",
+           f"-Wl,-rpath,{lib.parent}"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"tool build failed: {' '.join(cmd)}\n{res.stderr}")
+    return [exe]
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--force", action="store_true")
@@ -170,6 +185,7 @@ def main() -> None:
                 defines=tuple(args.define), out_dir=args.out_dir)
     if args.out_dir is None:
         build_c_tests(lib)
+        build_tools(lib)
     print(lib)
 
 

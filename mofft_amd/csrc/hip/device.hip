@@ -176,6 +176,29 @@ int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
   return (int)hipGetLastError();
 }
 
+static hipEvent_t timer_events[2];
+static int timer_ready = 0;
+
+int mofft_hip_timer_begin(void *stream) {
+  if (!timer_ready) {
+    if (hipEventCreate(&timer_events[0]) != hipSuccess ||
+        hipEventCreate(&timer_events[1]) != hipSuccess)
+      return (int)hipErrorInvalidValue;
+    timer_ready = 1;
+  }
+  return (int)hipEventRecord(timer_events[0], (hipStream_t)stream);
+}
+
+int mofft_hip_timer_end(void *stream, double *seconds) {
+  if (!timer_ready || !seconds) return (int)hipErrorInvalidValue;
+  hipError_t e = hipEventRecord(timer_events[1], (hipStream_t)stream);
+  if (e == hipSuccess) e = hipEventSynchronize(timer_events[1]);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, timer_events[0], timer_events[1]);
+  *seconds = (double)ms * 1e-3;
+  return (int)e;
+}
+
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream) {
   if (kernel < 0 || kernel >= mofft::launch_table_size || !args)

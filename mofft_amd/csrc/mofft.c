@@ -935,12 +935,15 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
             mofft_hip_copy(dout, dref, bytes);
             mofft_hip_synchronize(NULL);
           }
-          const double t0 = monotonic_seconds();
-          st = execute_device(p, xin, dout, p->workspace,
-                              schedule_workspace(&p->s, placement), NULL);
-          if (st == MOFFT_SUCCESS && mofft_hip_synchronize(NULL))
+          /* device time of the launches (HIP events on the null stream) */
+          double secs = 0.0;
+          if (mofft_hip_timer_begin(NULL)) st = MOFFT_INTERNAL_ERROR;
+          if (st == MOFFT_SUCCESS)
+            st = execute_device(p, xin, dout, p->workspace,
+                                schedule_workspace(&p->s, placement), NULL);
+          if (st == MOFFT_SUCCESS && mofft_hip_timer_end(NULL, &secs))
             st = MOFFT_INTERNAL_ERROR;
-          times[it] = monotonic_seconds() - t0;
+          times[it] = secs;
         }
         c->samples[smp] = scalar_median(times, iterations);
       }

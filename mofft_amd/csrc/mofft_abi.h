@@ -87,6 +87,10 @@ int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
                         void *stream);
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream);
+/* Device-side interval timer on `stream` (HIP events): begin records the
+ * start, end records the stop, waits for it and returns seconds. */
+int mofft_hip_timer_begin(void *stream);
+int mofft_hip_timer_end(void *stream, double *seconds);
 
 #ifdef __cplusplus
 }

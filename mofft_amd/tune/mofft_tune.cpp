@@ -1,0 +1,210 @@
+/*
+ * mofft_tune - GPU autotuner that writes the wisdom MOFFT_FLAG_PRETUNED reads.
+ *
+ * Role of the reference's tools/plan_search.cpp (model shortlist, measured
+ * finalists, JSON wisdom, plan_search.cpp:100-639, 782-882) for the MI355X
+ * build: for every (length, precision) it runs the library's MEASURE planner
+ * over the GPU candidate space (pass decompositions x pass boundaries x
+ * kernel banks / tile shapes, timed with HIP events on a batch of transforms),
+ * re-times the winner and the analytical model plan in an interleaved steady
+ * phase, and records the faster one.  The JSON goes through
+ * mofft_compiler.wisdom into the library at the next build.
+ *
+ * usage: mofft_tune [--precision 32|64|both] [--lengths N,N,...]
+ *                   [--min-log2 A --max-log2 B] [--points P]
+ *                   [--candidates C] [--samples S] -o wisdom.json
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mofft_hip.h"
+
+namespace {
+
+struct Result {
+  size_t length;
+  int precision;
+  std::vector<int> radices;
+  uint64_t mask;
+  unsigned bank;
+  double seconds;
+  double model_seconds;
+  size_t batch;
+  size_t candidates;
+};
+
+void *plan_create(int bits, size_t n, size_t batch, const mofft_plan_options *o,
+                  mofft_plan_diagnostics *d, mofft_status *st) {
+  void *p = nullptr;
+  if (bits == 32)
+    *st = mofft_plan_create_batched_f32((mofft_plan_f32 **)&p, n, batch, MOFFT_FORWARD,
+                                        MOFFT_OUT_OF_PLACE, o, d);
+  else
+    *st = mofft_plan_create_batched_f64((mofft_plan_f64 **)&p, n, batch, MOFFT_FORWARD,
+                                        MOFFT_OUT_OF_PLACE, o, d);
+  return p;
+}
+
+void plan_destroy(int bits, void *p) {
+  if (bits == 32) mofft_plan_destroy_f32((mofft_plan_f32 *)p);
+  else mofft_plan_destroy_f64((mofft_plan_f64 *)p);
+}
+
+mofft_status run(int bits, void *p, void *in, void *out, hipStream_t s) {
+  if (bits == 32)
+    return mofft_execute_async_f32((const mofft_plan_f32 *)p, (const mofft_complex_f32 *)in,
+                                   (mofft_complex_f32 *)out, s);
+  return mofft_execute_async_f64((const mofft_plan_f64 *)p, (const mofft_complex_f64 *)in,
+                                 (mofft_complex_f64 *)out, s);
+}
+
+/* median device time of `reps` executions, each bracketed by events */
+double steady(int bits, void *p, void *in, void *out, hipStream_t s, int reps) {
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  std::vector<double> t;
+  for (int i = 0; i < 3; ++i) run(bits, p, in, out, s);
+  for (int i = 0; i < reps; ++i) {
+    hipEventRecord(a, s);
+    run(bits, p, in, out, s);
+    hipEventRecord(b, s);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    t.push_back(ms * 1e-3);
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  std::sort(t.begin(), t.end());
+  return t[t.size() / 2];
+}
+
+std::vector<size_t> parse_list(const char *s) {
+  std::vector<size_t> v;
+  while (*s) {
+    char *end = nullptr;
+    v.push_back(strtoull(s, &end, 10));
+    s = *end == ',' ? end + 1 : end;
+    if (end == s && *s) break;
+  }
+  return v;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::vector<int> precisions = {32, 64};
+  std::vector<size_t> lengths;
+  int lo = 4, hi = 20;
+  size_t points = (size_t)1 << 24;
+  size_t candidates = 32;
+  unsigned samples = 5;
+  const char *out_path = "wisdom.json";
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    auto next = [&]() { return i + 1 < argc ? argv[++i] : (char *)""; };
+    if (a == "--precision") {
+      std::string v = next();
+      precisions = v == "32" ? std::vector<int>{32} : v == "64" ? std::vector<int>{64} : std::vector<int>{32, 64};
+    } else if (a == "--lengths") lengths = parse_list(next());
+    else if (a == "--min-log2") lo = atoi(next());
+    else if (a == "--max-log2") hi = atoi(next());
+    else if (a == "--points") points = strtoull(next(), nullptr, 10);
+    else if (a == "--candidates") candidates = strtoull(next(), nullptr, 10);
+    else if (a == "--samples") samples = (unsigned)atoi(next());
+    else if (a == "-o") out_path = next();
+    else { fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
+  }
+  if (lengths.empty())
+    for (int k = lo; k <= hi; ++k) lengths.push_back((size_t)1 << k);
+  if (!mofft_device_available()) {
+    fprintf(stderr, "mofft_tune: no gfx950 device\n");
+    return 1;
+  }
+  hipStream_t stream = nullptr;
+  std::vector<Result> results;
+  for (int bits : precisions) {
+    const size_t cb = bits == 32 ? 8 : 16;
+    for (size_t n : lengths) {
+      const size_t batch = std::max<size_t>(1, points / n);
+      void *in = nullptr, *out = nullptr;
+      if (hipMalloc(&in, n * batch * cb) != hipSuccess ||
+          hipMalloc(&out, n * batch * cb) != hipSuccess) {
+        fprintf(stderr, "allocation failed for N=%zu\n", n);
+        return 1;
+      }
+      hipMemset(in, 0, n * batch * cb);
+      mofft_plan_options o;
+      mofft_plan_options_init(&o);
+      o.mode = MOFFT_PLANNER_MEASURE;
+      o.candidate_limit = candidates;
+      o.timing_samples = samples;
+      mofft_plan_diagnostics d;
+      mofft_status st;
+      void *measured = plan_create(bits, n, batch, &o, &d, &st);
+      if (st != MOFFT_SUCCESS) {
+        fprintf(stderr, "N=%zu f%d: %s\n", n, bits, mofft_status_string(st));
+        hipFree(in); hipFree(out);
+        continue;
+      }
+      mofft_plan_diagnostics dm;
+      void *model = plan_create(bits, n, batch, nullptr, &dm, &st);
+      /* interleaved steady timing of winner and model (plan_search.cpp:612-622) */
+      double tw = 0, tm = 0;
+      for (int round = 0; round < 3; ++round) {
+        tw += steady(bits, measured, in, out, stream, 15);
+        if (model) tm += steady(bits, model, in, out, stream, 15);
+      }
+      Result r;
+      r.length = n;
+      r.precision = bits;
+      r.batch = batch;
+      r.candidates = d.candidates_measured;
+      r.model_seconds = tm / 3;
+      const bool use_measured = !model || tw <= tm * 0.99;
+      const mofft_plan_diagnostics &win = use_measured ? d : dm;
+      r.radices.assign(win.radices, win.radices + win.radix_count);
+      r.mask = win.blocked_transpose_mask;
+      r.bank = win.kernel_bank;
+      r.seconds = use_measured ? tw / 3 : tm / 3;
+      results.push_back(r);
+      const double gbs = 2.0 * n * batch * cb / r.seconds / 1e9;
+      printf("f%d N=%zu batch=%zu: %s plan, %.3f us (model %.3f us), %.0f GB/s, %zu candidates\n",
+             bits, n, batch, use_measured ? "measured" : "model", r.seconds * 1e6,
+             r.model_seconds * 1e6, gbs, r.candidates);
+      fflush(stdout);
+      plan_destroy(bits, measured);
+      if (model) plan_destroy(bits, model);
+      hipFree(in);
+      hipFree(out);
+    }
+  }
+  FILE *f = fopen(out_path, "w");
+  if (!f) return 1;
+  fprintf(f, "{\n \"schema_version\": 1,\n \"target\": \"gfx950\",\n");
+  fprintf(f, " \"formal_measurement\": false,\n \"source\": \"mofft_tune\",\n");
+  fprintf(f, " \"build_id\": \"%s\",\n \"entries\": [\n", mofft_build_id());
+  for (size_t i = 0; i < results.size(); ++i) {
+    const Result &r = results[i];
+    fprintf(f, "  {\"length\": %zu, \"precision\": \"fp%d\", \"radices\": [", r.length, r.precision);
+    for (size_t k = 0; k < r.radices.size(); ++k)
+      fprintf(f, "%s%d", k ? ", " : "", r.radices[k]);
+    fprintf(f, "], \"blocked_transpose_mask\": %llu, \"kernel_bank\": %u, "
+               "\"direct_input\": false, \"stage_layout\": \"transpose\", "
+               "\"median_seconds\": %.9g, \"model_seconds\": %.9g, \"batch\": %zu}%s\n",
+            (unsigned long long)r.mask, r.bank, r.seconds, r.model_seconds, r.batch,
+            i + 1 < results.size() ? "," : "");
+  }
+  fprintf(f, " ]\n}\n");
+  fclose(f);
+  printf("wrote %zu entries to %s\n", results.size(), out_path);
+  return 0;
+}

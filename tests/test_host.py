@@ -104,18 +104,30 @@ def test_no_device_means_unsupported_hardware():
 
 
 @pytest.mark.parametrize("n,bits,passes", [
-    (1024, 32, [1024]), (8192, 32, [8192]), (4096, 64, [4096]),
-    (1 << 16, 64, [256, 256]), (1 << 20, 32, [1024, 1024]), (1 << 14, 32, [16384]), (1 << 15, 32, [32768]),
-    (1 << 14, 64, [16384]), (1 << 15, 64, [256, 128]),
     (1 << 24, 32, [256, 256, 256]), (1 << 24, 64, [256, 256, 256]),
     (45, 32, [45]), (20736, 64, [144, 144]), (371293, 32, [2197, 169]),
+    (1728, 64, [1728]), (759375, 32, [3375, 225]),
 ])
 def test_model_schedule(n, bits, passes):
+    """Lengths without a wisdom entry: the analytical planner's schedule
+    (fewest HBM passes, then the most balanced split)."""
     d = mf.describe(n, 1, bits)
     assert d.passes == passes
     assert math.prod(d.chain) == n
     # the mask marks exactly the pass boundaries
     assert bin(d.pass_mask).count("1") == len(passes) - 1
+
+
+@pytest.mark.parametrize("n,bits,count", [
+    (1024, 32, 1), (8192, 32, 1), (4096, 64, 1), (1 << 14, 32, 1), (1 << 15, 32, 1),
+    (1 << 14, 64, 1), (1 << 16, 64, 2), (1 << 20, 32, 2), (1 << 15, 64, 2),
+])
+def test_pretuned_pass_counts(n, bits, count):
+    """Single pass up to 32768 (FP32) / 16384 (FP64) points, two passes above
+    (whether the chain comes from compiled wisdom or the model)."""
+    d = mf.describe(n, 1, bits)
+    assert d.pass_count == count
+    assert math.prod(d.passes) == n and math.prod(d.chain) == n
 
 
 def test_unsupported_sizes():
@@ -218,3 +230,16 @@ def test_recipe_parser_rejections():
         assert _import(good) == (mf.INCOMPATIBLE_RECIPE, None)
     # embedded NUL / wrong size
     assert _import(good, size=len(good))[0] == mf.INVALID_ARGUMENT
+
+
+def test_compiled_wisdom_matches_json():
+    """PRETUNED reads the autotuner's wisdom (mofft_amd/wisdom/gfx950.json)
+    compiled into the library: every entry's chain and pass boundaries."""
+    import json
+    data = json.loads((ROOT / "mofft_amd" / "wisdom" / "gfx950.json").read_text())
+    assert mf.lib().mofft_wisdom_source().decode() == data.get("source", "none")
+    for e in data["entries"]:
+        bits = 32 if e["precision"] == "fp32" else 64
+        d = mf.describe(e["length"], 1, bits)
+        assert d.chain == e["radices"], e
+        assert d.pass_mask == e["blocked_transpose_mask"], e
