@@ -19,8 +19,11 @@ Prints ONE JSON line:
              on a bounded sample of the same workload (FFTW 3.3.11, the
              comparison the reference uses, is not available offline).
 
-Other workloads (--workload c1|c3) print their own lines for study; the
-driver's line is the default.
+Other workloads (--workload c1|c3|c4) print their own lines for study; the
+driver's line is the default.  c4: 4096 FP32 transforms of N = 2^U{6..18}
+(seeded), half forward, half backward, mixed placement, in one persistent
+launch replayed from a captured HIP graph, beside the same transforms
+launched one plan execution each.
 """
 from __future__ import annotations
 
@@ -42,7 +45,10 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "copy"])
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-log2", default="6,18",
+                    help="c4: lengths 2^U{a..b}")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweep-max", type=int, default=20)
@@ -153,6 +159,8 @@ def run_gpu(args):
         print(json.dumps({"workload": "copy calibration", "bytes_read": nbytes,
                           "bytes_written": nbytes, "copy_gbs": gbs}), flush=True)
         return None, None
+    elif args.workload == "c4":
+        return run_batch(args)
     else:
         return run_sweep(args)
 
@@ -228,6 +236,83 @@ def run_gpu(args):
     for p, _, _ in plans:
         p.destroy()
     return line, (n, bits)
+
+
+def run_batch(args):
+    """Uneven batch in one persistent launch (mofft_batch_*)."""
+    import numpy as np
+    import torch
+    import mofft_amd as mf
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sh = stream.cuda_stream
+    g = torch.Generator(device=dev)
+    g.manual_seed(4096)
+    rng = np.random.default_rng(2026)
+    count = args.batch_count
+    ka, kb = (int(v) for v in args.batch_log2.split(","))
+    lengths = [1 << int(k) for k in rng.integers(ka, kb + 1, count)]
+    dirs = [mf.FORWARD if d else mf.BACKWARD for d in rng.integers(0, 2, count)]
+    inplace = [bool(v) for v in rng.integers(0, 2, count)]
+    ins = [torch.view_as_complex(torch.randn(n, 2, device=dev, generator=g)) for n in lengths]
+    outs = [a if ip else torch.empty_like(a) for a, ip in zip(ins, inplace)]
+    bp = mf.BatchPlan(lengths, dirs, ins, outs, 32, graph=True)
+    d = bp.description()
+
+    def timed(fn, reps):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        es = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        for a, b in es:
+            a.record(stream)
+            fn()
+            b.record(stream)
+        torch.cuda.synchronize()
+        return [a.elapsed_time(b) for a, b in es]
+
+    for _ in range(args.warmup):
+        bp.execute_async(sh)
+    t = timed(lambda: bp.execute_async(sh), args.steps)
+    bp.check()
+    ms = sum(t) / len(t)
+    # the same transforms, one single-transform plan execution each
+    plans = {}
+    for n, dr, ip in set(zip(lengths, dirs, inplace)):
+        plans[(n, dr, ip)] = mf.Plan(n, 1, 32, dr, in_place=ip)
+
+    def each():
+        for i, n in enumerate(lengths):
+            plans[(n, dirs[i], inplace[i])].execute_async(ins[i], outs[i], sh)
+    te = timed(each, 3)
+    ms_each = sum(te) / len(te)
+    gbs = d.algorithmic_bytes / (ms * 1e-3) / 1e9
+    line = {
+        "metric": "uneven batch GFLOP/s (5N log2N) in one persistent launch",
+        "value": round(d.flops / (ms * 1e-3) / 1e9, 2), "unit": "GFLOP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms, 4), "p50_ms": round(percentile(t, 0.5), 4),
+        "p99_ms": round(percentile(t, 0.99), 4), "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic (seeded on device)",
+        "config": {"workload": f"{count} fp32 transforms N=2^U{{{ka}..{kb}}} 50/50 fwd/bwd "
+                               "mixed placement, one persistent launch (hipGraph)",
+                   "jobs": count, "distinct_lengths": d.distinct_lengths,
+                   "multi_pass_jobs": d.multi_pass_jobs, "shared_jobs": d.shared_jobs,
+                   "items": d.item_count,
+                   "segments": d.segment_count, "grid": d.grid,
+                   "blocks_per_cu": d.blocks_per_cu, "graph": bool(d.graph)},
+        "hbm_gbs": round(gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)},
+        "per_transform_launches_ms": round(ms_each, 4),
+        "speedup_vs_per_transform": round(ms_each / ms, 2),
+    }
+    bp.destroy()
+    for p in plans.values():
+        p.destroy()
+    print(json.dumps(line), flush=True)
+    return None, None
 
 
 def run_sweep(args):

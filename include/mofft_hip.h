@@ -8,7 +8,9 @@
  *     back (transform b starts at element b*length);
  *   - stream-ordered execution on device pointers with no host sync;
  *   - a caller-supplied workspace so one plan can run on several streams;
- *   - host-only plan description (no device needed) for tooling and tests.
+ *   - host-only plan description (no device needed) for tooling and tests;
+ *   - persistent uneven batches: many transforms of different lengths and
+ *     directions in one launch fed by a device work queue.
  * Streams are passed as `void *` holding a hipStream_t (NULL = the null
  * stream), so this header needs no HIP include.
  */
@@ -123,6 +125,88 @@ mofft_status mofft_kernel_get_info(int index, mofft_kernel_info *info);
 
 /* 1 when a gfx950 device is present and usable, else 0. */
 int mofft_device_available(void);
+
+/* ---- persistent uneven batch ------------------------------------------
+ *
+ * No counterpart in the reference, which executes one transform per call
+ * (reference src/mofft.c:2169-2201).  A batch plan binds `count` transforms
+ * of one precision, each with its own length, direction and device
+ * buffers (in-place when input == output), and runs all of them in ONE
+ * persistent kernel launch fed by a device-side work queue.  Plans,
+ * twiddles, workspace and the queue stay resident on the device; by default
+ * the launch is captured once into a HIP graph that every execution
+ * replays.  Lengths: powers of two >= 2 (one row tile up to 4096 points, two
+ * strided passes up to 2^18, three beyond).  A multi-pass transform is run
+ * whole by one workgroup unless it exceeds twice a fair share of the
+ * launch; then its pass tiles spread over workgroups that synchronise
+ * through device counters.  The radix chain of each length
+ * is the compiled wisdom's when its passes exist as batch tile kinds, else
+ * the analytical planner's over those kinds. */
+typedef struct mofft_batch_plan mofft_batch_plan;
+
+#define MOFFT_BATCH_NO_GRAPH 1u   /* launch directly, no captured graph */
+#define MOFFT_BATCH_SHARE_ALL 2u  /* every multi-pass transform spreads its
+                                     pass tiles over many workgroups (by
+                                     default only transforms above twice a
+                                     fair share of the launch do) */
+
+typedef struct {
+  uint32_t flags;        /* MOFFT_BATCH_* */
+  uint32_t grid;         /* workgroups; 0 = every resident slot */
+  uint64_t spin_limit;   /* polls before a dependency wait gives up and
+                            the execution reports MOFFT_INTERNAL_ERROR;
+                            0 = default (about a second) */
+} mofft_batch_options;
+
+typedef struct {
+  size_t job_count;
+  size_t distinct_lengths;
+  size_t multi_pass_jobs;
+  size_t shared_jobs;         /* multi-pass transforms run as shared tiles */
+  size_t wisdom_lengths;      /* lengths whose chain came from wisdom */
+  size_t segment_count;
+  size_t item_count;          /* queue items of one execution */
+  size_t grid;
+  size_t blocks_per_cu;
+  size_t lds_bytes;
+  size_t workspace_bytes;
+  size_t twiddle_bytes;
+  double flops;               /* sum of 5 N log2 N */
+  double algorithmic_bytes;   /* sum of 2 N sizeof(complex) */
+  int precision_bits;
+  int graph;                  /* 1: executions replay a captured graph */
+} mofft_batch_description;
+
+void mofft_batch_options_init(mofft_batch_options *options);
+/* directions[i] is MOFFT_FORWARD or MOFFT_BACKWARD; inputs/outputs are
+ * device pointers of lengths[i] complex values. */
+mofft_status mofft_batch_plan_create(mofft_batch_plan **plan,
+                                     int precision_bits, size_t count,
+                                     const size_t *lengths,
+                                     const int *directions,
+                                     const void *const *inputs,
+                                     void *const *outputs,
+                                     const mofft_batch_options *options);
+/* Rebind the buffers (same placements); no execution may be in flight. */
+mofft_status mofft_batch_set_buffers(mofft_batch_plan *plan,
+                                     const void *const *inputs,
+                                     void *const *outputs);
+/* Enqueue every transform on `stream`.  Executions of one plan share its
+ * queue and must be ordered on one stream. */
+mofft_status mofft_batch_execute_async(const mofft_batch_plan *plan,
+                                       void *stream);
+/* After the stream's executions completed: MOFFT_INTERNAL_ERROR if one of
+ * them gave up on a dependency wait, else MOFFT_SUCCESS. */
+mofft_status mofft_batch_check(const mofft_batch_plan *plan);
+/* Execute on the null stream, wait, and check. */
+mofft_status mofft_batch_execute(const mofft_batch_plan *plan);
+mofft_status mofft_batch_describe(const mofft_batch_plan *plan,
+                                  mofft_batch_description *description);
+/* Host only: the schedule a batch plan uses for one length (pass_kernel
+ * holds batch tile-kind indices). */
+mofft_status mofft_batch_describe_length(int precision_bits, size_t length,
+                                         mofft_plan_description *description);
+void mofft_batch_plan_destroy(mofft_batch_plan *plan);
 
 #ifdef __cplusplus
 }

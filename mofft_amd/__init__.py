@@ -117,6 +117,26 @@ class KernelInfo(ctypes.Structure):
                 ("stages", ctypes.c_int * 8)]
 
 
+class BatchOptions(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_uint32), ("grid", ctypes.c_uint32),
+                ("spin_limit", ctypes.c_uint64)]
+
+
+class BatchDescription(ctypes.Structure):
+    _fields_ = [("job_count", ctypes.c_size_t), ("distinct_lengths", ctypes.c_size_t),
+                ("multi_pass_jobs", ctypes.c_size_t), ("shared_jobs", ctypes.c_size_t),
+                ("wisdom_lengths", ctypes.c_size_t),
+                ("segment_count", ctypes.c_size_t), ("item_count", ctypes.c_size_t),
+                ("grid", ctypes.c_size_t), ("blocks_per_cu", ctypes.c_size_t),
+                ("lds_bytes", ctypes.c_size_t), ("workspace_bytes", ctypes.c_size_t),
+                ("twiddle_bytes", ctypes.c_size_t), ("flops", ctypes.c_double),
+                ("algorithmic_bytes", ctypes.c_double), ("precision_bits", ctypes.c_int),
+                ("graph", ctypes.c_int)]
+
+
+BATCH_NO_GRAPH = 1
+BATCH_SHARE_ALL = 2
+
 _lib = None
 
 # (name, restype, argtypes); P = void pointer
@@ -177,6 +197,17 @@ _SIGS += [
     ("mofft_kernel_count", _I, []),
     ("mofft_kernel_get_info", _I, [_I, ctypes.POINTER(KernelInfo)]),
     ("mofft_device_available", _I, []),
+    ("mofft_batch_options_init", None, [ctypes.POINTER(BatchOptions)]),
+    ("mofft_batch_plan_create", _I,
+     [ctypes.POINTER(_P), _I, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(_I),
+      ctypes.POINTER(_P), ctypes.POINTER(_P), ctypes.POINTER(BatchOptions)]),
+    ("mofft_batch_set_buffers", _I, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    ("mofft_batch_execute_async", _I, [_P, _P]),
+    ("mofft_batch_check", _I, [_P]),
+    ("mofft_batch_execute", _I, [_P]),
+    ("mofft_batch_describe", _I, [_P, ctypes.POINTER(BatchDescription)]),
+    ("mofft_batch_describe_length", _I, [_I, _SZ, ctypes.POINTER(PlanDescription)]),
+    ("mofft_batch_plan_destroy", None, [_P]),
 ]
 
 
@@ -344,6 +375,85 @@ class Plan:
     def destroy(self) -> None:
         if self._handle and self._handle.value:
             self._fn("mofft_plan_destroy")(self._handle)
+            self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+def batch_describe_length(length: int, precision_bits: int = 32) -> PlanDescription:
+    """Host-only schedule a batch plan uses for one length."""
+    d = PlanDescription()
+    _check(lib().mofft_batch_describe_length(precision_bits, length, ctypes.byref(d)),
+           f"batch_describe_length N={length}")
+    return d
+
+
+class BatchPlan:
+    """Many transforms of uneven length in one persistent launch
+    (mofft_batch_plan_create).  `inputs`/`outputs` are device buffers
+    (torch tensors or raw pointers); a transform is in place when its input
+    and output are the same buffer."""
+
+    def __init__(self, lengths: Sequence[int], directions: Sequence[int],
+                 inputs, outputs, precision_bits: int = 32, graph: bool = True,
+                 grid: int = 0, spin_limit: int = 0, share_all: bool = False):
+        n = len(lengths)
+        if not (len(directions) == len(inputs) == len(outputs) == n):
+            raise ValueError("lengths, directions, inputs and outputs differ in size")
+        self.precision_bits = precision_bits
+        self.count = n
+        self._lengths = (ctypes.c_size_t * n)(*[int(x) for x in lengths])
+        self._dirs = (ctypes.c_int * n)(*[int(d) for d in directions])
+        ins, outs = self._pointers(inputs, outputs)
+        opts = BatchOptions()
+        lib().mofft_batch_options_init(ctypes.byref(opts))
+        opts.flags = (0 if graph else BATCH_NO_GRAPH) | (BATCH_SHARE_ALL if share_all else 0)
+        opts.grid = grid
+        opts.spin_limit = spin_limit
+        self._handle = ctypes.c_void_p()
+        _check(lib().mofft_batch_plan_create(ctypes.byref(self._handle), precision_bits, n,
+                                             self._lengths, self._dirs, ins, outs,
+                                             ctypes.byref(opts)),
+               f"batch plan of {n} transforms f{precision_bits}")
+
+    def _pointers(self, inputs, outputs):
+        n = self.count
+        ins = (ctypes.c_void_p * n)(*[x if isinstance(x, int) else _ptr(x) for x in inputs])
+        outs = (ctypes.c_void_p * n)(*[x if isinstance(x, int) else _ptr(x) for x in outputs])
+        return ins, outs
+
+    def set_buffers(self, inputs, outputs) -> None:
+        ins, outs = self._pointers(inputs, outputs)
+        _check(lib().mofft_batch_set_buffers(self._handle, ins, outs), "batch_set_buffers")
+
+    def execute_async(self, stream: int = 0) -> None:
+        _check(lib().mofft_batch_execute_async(self._handle, stream or None),
+               "batch_execute_async")
+
+    def execute(self) -> None:
+        _check(lib().mofft_batch_execute(self._handle), "batch_execute")
+
+    def check(self) -> None:
+        _check(lib().mofft_batch_check(self._handle), "batch_check")
+
+    def description(self) -> BatchDescription:
+        d = BatchDescription()
+        _check(lib().mofft_batch_describe(self._handle, ctypes.byref(d)), "batch_describe")
+        return d
+
+    def destroy(self) -> None:
+        if self._handle and self._handle.value:
+            lib().mofft_batch_plan_destroy(self._handle)
             self._handle = ctypes.c_void_p()
 
     def __del__(self):

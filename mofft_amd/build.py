@@ -49,10 +49,12 @@ def _generate() -> None:
 
 
 def _sources() -> tuple[list[Path], list[Path], list[Path]]:
-    c_srcs = [CSRC / "mofft.c", GEN / "mofft_registry.c", GEN / "mofft_wisdom.c",
+    c_srcs = [CSRC / "mofft.c", CSRC / "mofft_batch.c", GEN / "mofft_registry.c",
+              GEN / "mofft_wisdom.c",
               GEN / "mofft_launch_table.cpp"]
     hip_srcs = [CSRC / "hip" / "device.hip"]
     hip_srcs += sorted(GEN.glob("mofft_kernels_p*.hip"))
+    hip_srcs += sorted(GEN.glob("mofft_batch_f*.hip"))
     headers = sorted(CSRC.glob("*.h")) + sorted((CSRC / "hip").glob("*.h")) + \
         sorted(GEN.glob("*.h")) + sorted((ROOT / "include").glob("*.h"))
     return c_srcs, hip_srcs, headers

@@ -90,14 +90,48 @@ def _targs(e: registry.Entry) -> str:
     return f"{t}, {e.length}, {e.tpt}, {e.cols}, {'true' if e.rows else 'false'}, {stages}"
 
 
-def emit_registry_c(es: list[registry.Entry]) -> str:
-    lines = [HEADER, '#include "../mofft_abi.h"', "",
-             "const mofft_registry_entry mofft_registry[] = {"]
+def _registry_rows(es: list[registry.Entry]) -> list[str]:
+    lines = []
     for e in es:
         st = ", ".join(str(r) for r in e.stages)
         lines.append(f"  {{{e.precision}, {e.length}, {e.rows}, {e.tpt}, {e.cols}, "
                      f"{len(e.stages)}, {{{st}}}, {e.lds_bytes}, {e.variant}}},")
-    lines += ["};", f"const int mofft_registry_count = {len(es)};", ""]
+    return lines
+
+
+def emit_registry_c(es: list[registry.Entry], bs: list[registry.Entry]) -> str:
+    lines = [HEADER, '#include "../mofft_abi.h"', "",
+             "const mofft_registry_entry mofft_registry[] = {"]
+    lines += _registry_rows(es)
+    lines += ["};", f"const int mofft_registry_count = {len(es)};", "",
+              "/* tile kinds of the persistent batch kernels (index = kind) */",
+              "const mofft_registry_entry mofft_batch_registry[] = {"]
+    lines += _registry_rows(bs)
+    lines += ["};", f"const int mofft_batch_registry_count = {len(bs)};", ""]
+    return "\n".join(lines)
+
+
+def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
+    """One precision's persistent batch kernel: a switch over its kinds."""
+    t = "float" if precision == 32 else "double"
+    lines = [HEADER, '#include "../hip/batch.h"', "", "namespace mofft {",
+             f"struct BatchDispatchF{precision} {{",
+             f"  static __device__ __forceinline__ void run(int kind, const mofft_batch_segment &sg,",
+             f"                                             int64_t block, {t} *lds) {{",
+             "    switch (kind) {"]
+    for i, e in enumerate(bs):
+        if e.precision != precision:
+            continue
+        lines.append(f"      case {i}: batch_tile<{_targs(e)}>(sg, block, lds); return;")
+    lines += ["      default: return;", "    }", "  }", "};", "",
+              f"__global__ void __launch_bounds__(kBatchThreads)",
+              f"batch_kernel_f{precision}(mofft_batch_header h) {{",
+              f"  batch_loop<{t}, BatchDispatchF{precision}>(h);", "}", "",
+              f"int batch_occupancy_f{precision}(int lds, int *bpc, int *cus) {{",
+              f"  return BatchLauncher<batch_kernel_f{precision}>::occupancy(lds, bpc, cus);", "}",
+              f"int batch_launch_f{precision}(const mofft_batch_header *h, int grid, int lds, void *s) {{",
+              f"  return BatchLauncher<batch_kernel_f{precision}>::launch(h, grid, lds, (hipStream_t)s);",
+              "}", "}  // namespace mofft", ""]
     return "\n".join(lines)
 
 
@@ -132,7 +166,10 @@ def emit(out_dir: str | Path, parts: int = 8) -> dict:
     registry.check(es)
     files: dict[str, str] = {}
     files["mofft_codelets.h"], counts = emit_codelets_header()
-    files["mofft_registry.c"] = emit_registry_c(es)
+    bs = registry.batch_entries()
+    files["mofft_registry.c"] = emit_registry_c(es, bs)
+    for precision in (32, 64):
+        files[f"mofft_batch_f{precision}.hip"] = emit_batch_part(bs, precision)
     # balance parts by an estimate of compile cost (threads * stages)
     order = sorted(range(len(es)), key=lambda i: -(es[i].length * len(es[i].stages)))
     buckets: list[list[int]] = [[] for _ in range(parts)]
@@ -156,6 +193,10 @@ def emit(out_dir: str | Path, parts: int = 8) -> dict:
                      "stages": list(e.stages), "lds_bytes": e.lds_bytes,
                      "variant": e.variant}
                     for e in es],
+        "batch_kinds": [{"precision": e.precision, "length": e.length, "rows": e.rows,
+                         "threads_per_column": e.tpt, "columns_per_block": e.cols,
+                         "stages": list(e.stages), "lds_bytes": e.lds_bytes}
+                        for e in bs],
         "codelet_ops": {str(k): v for k, v in counts.items()},
         "digest": digest.hexdigest(),
     }

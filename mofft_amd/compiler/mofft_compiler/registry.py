@@ -189,6 +189,49 @@ def entries() -> list[Entry]:
     return out
 
 
+# Persistent batch kernel (csrc/hip/batch.h): every tile kind runs with the
+# same 256-thread workgroup.  Whole-transform (row) kinds cover power-of-two
+# lengths up to 4096, strided kinds up to 512 (so every power of two up to
+# 2^18 is one or two passes; longer ones take three).
+BATCH_THREADS = 256
+BATCH_ROWS_MAX = 4096
+BATCH_COLS_MAX = 512
+
+
+def _pow2_floor(x: int) -> int:
+    return 1 << (max(1, x).bit_length() - 1)
+
+
+def batch_entries() -> list[Entry]:
+    """Tile kinds of the persistent batch kernel: the bank-0 chains of the
+    power-of-two entries, with threads per column a power of two and
+    columns * threads = 256.  Strided kinds keep at least 16 (FP32) / 8
+    (FP64) adjacent columns, i.e. 128-byte runs per access, unless that
+    leaves a thread more than 16 points of a sub-stage (register pressure
+    of the one kernel is that of its largest kind)."""
+    out: list[Entry] = []
+
+    def points_per_thread(length, stages, tpt):
+        return max(-(-(length // r) // tpt) * r for r in stages)
+
+    for precision in (32, 64):
+        cmin = 16 if precision == 32 else 8
+        for rows, lmax in ((1, BATCH_ROWS_MAX), (0, BATCH_COLS_MAX)):
+            for k in range(1, lmax.bit_length()):
+                length = 1 << k
+                stages = _pow2_chain(length)
+                tpt = min(BATCH_THREADS, _pow2_floor(length // max(stages)))
+                if not rows:
+                    tpt = min(tpt, BATCH_THREADS // cmin)
+                    while (points_per_thread(length, stages, tpt) > 16 and
+                           2 * tpt <= BATCH_THREADS):
+                        tpt *= 2
+                e = Entry(precision, length, rows, tpt, BATCH_THREADS // tpt, stages)
+                assert e.threads == BATCH_THREADS and e.lds_bytes <= LDS_CAP, e
+                out.append(e)
+    return out
+
+
 def check(es: list[Entry]) -> None:
     for e in es:
         prod = 1
@@ -214,5 +257,5 @@ def describe(es: list[Entry]) -> str:
     return "\n".join(lines)
 
 
-__all__ = ["Entry", "entries", "check", "describe", "chain_for"]
+__all__ = ["Entry", "entries", "batch_entries", "check", "describe", "chain_for"]
 _ = itertools  # kept for callers that enumerate chains

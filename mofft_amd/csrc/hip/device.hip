@@ -17,6 +17,11 @@ namespace mofft {
 typedef int (*launch_fn)(const mofft_pass_args *, void *);
 extern const launch_fn launch_table[];
 extern const int launch_table_size;
+/* persistent batch kernels (generated mofft_batch_f32/f64.hip) */
+int batch_occupancy_f32(int lds, int *bpc, int *cus);
+int batch_occupancy_f64(int lds, int *bpc, int *cus);
+int batch_launch_f32(const mofft_batch_header *h, int grid, int lds, void *s);
+int batch_launch_f64(const mofft_batch_header *h, int grid, int lds, void *s);
 
 /* exp(-2*pi*i*m/M) in FP64: exact integer octant reduction of 8m/8M, then
  * sincospi of an argument in [0, 1/4]. */
@@ -204,6 +209,63 @@ int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
   if (kernel < 0 || kernel >= mofft::launch_table_size || !args)
     return (int)hipErrorInvalidValue;
   return mofft::launch_table[kernel](args, stream);
+}
+
+int mofft_hip_batch_occupancy(int precision_bits, int lds_bytes,
+                              int *blocks_per_cu, int *cu_count) {
+  if (!blocks_per_cu || !cu_count || lds_bytes < 0)
+    return (int)hipErrorInvalidValue;
+  if (precision_bits == 32)
+    return mofft::batch_occupancy_f32(lds_bytes, blocks_per_cu, cu_count);
+  if (precision_bits == 64)
+    return mofft::batch_occupancy_f64(lds_bytes, blocks_per_cu, cu_count);
+  return (int)hipErrorInvalidValue;
+}
+
+int mofft_hip_batch_launch(int precision_bits, const mofft_batch_header *h,
+                           int grid, int lds_bytes, void *stream) {
+  if (!h || grid <= 0 || lds_bytes < 0) return (int)hipErrorInvalidValue;
+  if (precision_bits == 32)
+    return mofft::batch_launch_f32(h, grid, lds_bytes, stream);
+  if (precision_bits == 64)
+    return mofft::batch_launch_f64(h, grid, lds_bytes, stream);
+  return (int)hipErrorInvalidValue;
+}
+
+/* Stream capture on a private stream; the graph holds two kernel nodes
+ * (counter reset, persistent kernel) and no memset node. */
+int mofft_hip_batch_graph(int precision_bits, const mofft_batch_header *h,
+                          int grid, int lds_bytes, void **graph_exec) {
+  if (!graph_exec) return (int)hipErrorInvalidValue;
+  *graph_exec = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return (int)e;
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    const int rc = mofft_hip_batch_launch(precision_bits, h, grid, lds_bytes, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    e = rc != 0 ? (hipError_t)rc : ec;
+    if (e == hipSuccess) {
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e == hipSuccess) *graph_exec = (void *)x;
+    }
+    if (g) (void)hipGraphDestroy(g);
+  }
+  (void)hipStreamDestroy(s);
+  return (int)e;
+}
+
+int mofft_hip_graph_launch(void *graph_exec, void *stream) {
+  if (!graph_exec) return (int)hipErrorInvalidValue;
+  return (int)hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream);
+}
+
+int mofft_hip_graph_destroy(void *graph_exec) {
+  if (!graph_exec) return 0;
+  return (int)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
 }
 
 }  // extern "C"

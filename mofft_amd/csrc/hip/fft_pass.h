@@ -105,9 +105,11 @@ struct Ctx {
   int64_t stride;
   int64_t ns;
   int64_t obase;      /* (j / ns) * ns * L + j % ns */
+  int64_t col0;       /* first column of this tile */
   uint64_t jm_scale;  /* (j % ns) * tw_scale */
   uint64_t split_mask;
   uint32_t split_shift;
+  int tid;            /* thread index in the workgroup */
   int c;
   int t;
   bool live;
@@ -203,7 +205,7 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
   constexpr int NB = L / R;
   constexpr bool FULL = IT * TPT == NB;
   constexpr int PER = (C * L) / TL::NT;
-  const int tid = threadIdx.x;
+  const int tid = x.tid;
   const int64_t ns = p.ns;
   const int span = (int)ns * L;
   int ad[PER];
@@ -243,7 +245,7 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
     }
   }
   cx<T> *out = (cx<T> *)p.out;
-  const int64_t col0 = (int64_t)blockIdx.x * C;
+  const int64_t col0 = x.col0;
   const int64_t b0 = col0 / p.stride;
   const int64_t j0 = col0 - b0 * p.stride;
   const bool swap = (p.flags & MOFFT_PASS_SWAP_OUT) != 0;
@@ -434,16 +436,16 @@ struct Stage {
   }
 };
 
-template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
-__global__ void __launch_bounds__(TPT * C)
-fft_pass_kernel(mofft_pass_args p) {
-  extern __shared__ __align__(16) unsigned char mofft_lds[];
-  Ctx<T> x;
-  x.lds = reinterpret_cast<T *>(mofft_lds);
-  const int tid = threadIdx.x;
+/* Per-thread context of tile `block` (C columns) of a pass. */
+template <typename T, int L, int TPT, int C, bool ROWS>
+__device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
+                                           int64_t block, T *lds, int tid) {
+  x.lds = lds;
+  x.tid = tid;
   x.c = ROWS ? tid / TPT : tid % C;
   x.t = ROWS ? tid % TPT : tid / C;
-  const int64_t col = (int64_t)blockIdx.x * C + x.c;
+  x.col0 = block * C;
+  const int64_t col = x.col0 + x.c;
   x.live = col < p.columns;
   const int64_t b = ROWS ? col : col / p.stride;
   const int64_t j = ROWS ? 0 : col - b * p.stride;
@@ -465,11 +467,28 @@ fft_pass_kernel(mofft_pass_args p) {
   x.pass_tw = (p.flags & MOFFT_PASS_TWIDDLE) != 0 && jr != 0;
   x.pass_table = (p.flags & MOFFT_PASS_TWIDDLE_TABLE) != 0;
   x.tpass = (const cx<T> *)p.tw_pass + jr;
+}
+
+/* Load, transform and store the tile described by x. */
+template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
+__device__ __forceinline__ void run_tile(const Ctx<T> &x,
+                                         const mofft_pass_args &p) {
   using First = Stage<T, L, TPT, C, ROWS, 1, 0, R0, R...>;
   T vr[First::IT][R0];
   T vi[First::IT][R0];
   First::load_first(x, vr, vi);
   First::run(x, p, vr, vi);
+}
+
+template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
+__global__ void __launch_bounds__(TPT * C)
+fft_pass_kernel(mofft_pass_args p) {
+  extern __shared__ __align__(16) unsigned char mofft_lds[];
+  Ctx<T> x;
+  setup_tile<T, L, TPT, C, ROWS>(x, p, (int64_t)blockIdx.x,
+                                 reinterpret_cast<T *>(mofft_lds),
+                                 (int)threadIdx.x);
+  run_tile<T, L, TPT, C, ROWS, R0, R...>(x, p);
 }
 
 template <typename T, int L, int TPT, int C, bool ROWS, int... R>

@@ -18,9 +18,7 @@
  *                               (the GPU meaning of the reference's
  *                               blocked-transpose mask, include/mofft.h:152)
  */
-#include "mofft_abi.h"
-#include "../../include/mofft.h"
-#include "../../include/mofft_hip.h"
+#include "mofft_internal.h"
 #include "mofft_wisdom.h"
 
 #include <float.h>
@@ -68,42 +66,6 @@ static size_t chunk_bytes(void) {
 /* src/mofft.c:27-28 - the radix set the API accepts. */
 static const int k_radices[] = {64, 32, 16, 15, 14, 13, 12, 11, 10,
                                 9, 8, 7, 6, 5, 4, 3, 2};
-
-typedef struct {
-  int kernel;            /* registry index */
-  size_t length;         /* points per column in LDS */
-  size_t stride;         /* n / length */
-  size_t span;           /* product of earlier pass lengths (ns) */
-  size_t stage_offset;   /* element offset of this pass's sub-stage table */
-  size_t stage_count;
-  size_t pass_offset;    /* element offset of the full pass twiddle table */
-  int pass_table;        /* 1: the full table exists for this pass */
-} mofft_pass;
-
-/* Passes after the first multiply by w_(ns*L)^((j%ns)*k).  Up to this many
- * bytes per pass the factors come from a full table laid out so adjacent
- * columns read adjacent entries (one coalesced load per point, L2/Infinity
- * Cache resident); beyond it from a two-level table (two gathered loads and
- * a complex product per point). */
-#define MOFFT_PASS_TABLE_BYTES ((size_t)8 << 20)
-
-typedef struct {
-  size_t length;
-  size_t batch;
-  int precision_bits;
-  unsigned int bank;
-  int depth;
-  int factors[MOFFT_MAX_PLAN_DEPTH];   /* outer -> inner */
-  uint64_t pass_mask;
-  int pass_count;
-  mofft_pass passes[MOFFT_MAX_PLAN_DEPTH];
-  size_t chunk;
-  size_t workspace_bytes;
-  uint32_t split_shift;
-  size_t tw_hi_offset, tw_hi_count;
-  size_t tw_lo_offset, tw_lo_count;
-  size_t twiddle_count;                /* complex elements */
-} mofft_schedule;
 
 typedef struct mofft_plan_common {
   mofft_schedule s;
@@ -172,10 +134,17 @@ int mofft_device_available(void) { return mofft_hip_device_available(); }
 /* kernel registry lookups                                              */
 /* ------------------------------------------------------------------ */
 
-static int find_kernel_variant(int precision_bits, int rows, const int *stages,
-                               int count, int variant) {
-  for (int i = 0; i < mofft_registry_count; ++i) {
-    const mofft_registry_entry *e = &mofft_registry[i];
+const mofft_kernel_set mofft_pass_kernel_set = {mofft_registry,
+                                                &mofft_registry_count};
+const mofft_kernel_set mofft_batch_kernel_set = {mofft_batch_registry,
+                                                 &mofft_batch_registry_count};
+#define PASS_KERNELS (&mofft_pass_kernel_set)
+
+static int find_kernel_variant(const mofft_kernel_set *ks, int precision_bits,
+                               int rows, const int *stages, int count,
+                               int variant) {
+  for (int i = 0; i < *ks->count; ++i) {
+    const mofft_registry_entry *e = &ks->entries[i];
     if (e->precision_bits != precision_bits || e->rows != rows ||
         e->variant != variant)
       continue;
@@ -184,8 +153,8 @@ static int find_kernel_variant(int precision_bits, int rows, const int *stages,
       return i;
   }
   if (count == 1) {
-    for (int i = 0; i < mofft_registry_count; ++i) {
-      const mofft_registry_entry *e = &mofft_registry[i];
+    for (int i = 0; i < *ks->count; ++i) {
+      const mofft_registry_entry *e = &ks->entries[i];
       if (e->precision_bits == precision_bits && e->rows == rows &&
           e->variant == variant && e->length == stages[0])
         return i;
@@ -197,18 +166,20 @@ static int find_kernel_variant(int precision_bits, int rows, const int *stages,
 /* Registry entry whose sub-stage chain equals stages[0..count) (execution
  * order), or, for a single radix, any entry of that length; the bank's
  * variant when compiled, else variant 0. */
-static int find_kernel(int precision_bits, int rows, const int *stages,
-                       int count, unsigned int bank) {
+static int find_kernel(const mofft_kernel_set *ks, int precision_bits,
+                       int rows, const int *stages, int count,
+                       unsigned int bank) {
   int k = -1;
   if (bank != 0)
-    k = find_kernel_variant(precision_bits, rows, stages, count, (int)bank);
-  if (k < 0) k = find_kernel_variant(precision_bits, rows, stages, count, 0);
+    k = find_kernel_variant(ks, precision_bits, rows, stages, count, (int)bank);
+  if (k < 0) k = find_kernel_variant(ks, precision_bits, rows, stages, count, 0);
   return k;
 }
 
-static int find_kernel_by_length(int precision_bits, int rows, size_t length) {
-  for (int i = 0; i < mofft_registry_count; ++i) {
-    const mofft_registry_entry *e = &mofft_registry[i];
+static int find_kernel_by_length(const mofft_kernel_set *ks, int precision_bits,
+                                 int rows, size_t length) {
+  for (int i = 0; i < *ks->count; ++i) {
+    const mofft_registry_entry *e = &ks->entries[i];
     if (e->precision_bits == precision_bits && e->rows == rows &&
         e->variant == 0 && (size_t)e->length == length)
       return i;
@@ -253,10 +224,11 @@ static mofft_status validate_chain(size_t length, const int *radices,
 /* Group the chain into passes.  Level boundaries named in `mask` always
  * split; elsewhere each pass takes the longest run of stages that a compiled
  * kernel implements (greedy from the innermost stage). */
-static mofft_status build_schedule(size_t length, size_t batch,
-                                   int precision_bits, const int *radices,
-                                   size_t depth, uint64_t mask,
-                                   unsigned int bank, mofft_schedule *s) {
+static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
+                                   size_t batch, int precision_bits,
+                                   const int *radices, size_t depth,
+                                   uint64_t mask, unsigned int bank,
+                                   mofft_schedule *s) {
   mofft_status st = validate_chain(length, radices, depth);
   if (st != MOFFT_SUCCESS) return st;
   if (batch == 0 || (precision_bits != 32 && precision_bits != 64))
@@ -265,6 +237,7 @@ static mofft_status build_schedule(size_t length, size_t batch,
     return MOFFT_INVALID_ARGUMENT;
   if (bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
   memset(s, 0, sizeof(*s));
+  s->ks = ks;
   s->length = length;
   s->batch = batch;
   s->precision_bits = precision_bits;
@@ -287,7 +260,7 @@ static mofft_status build_schedule(size_t length, size_t batch,
     for (size_t len = limit; len >= 1; --len) {
       const int rows = (start == 0 && len == depth) ? 1 : 0;
       if (len > 8) continue;
-      kernel = find_kernel(precision_bits, rows, exec + start, (int)len, bank);
+      kernel = find_kernel(ks, precision_bits, rows, exec + start, (int)len, bank);
       if (kernel >= 0) { take = len; break; }
     }
     if (kernel < 0) return MOFFT_UNSUPPORTED_SIZE;
@@ -329,7 +302,7 @@ static mofft_status build_schedule(size_t length, size_t batch,
   }
   for (int i = 0; i < s->pass_count; ++i) {
     mofft_pass *p = &s->passes[i];
-    const mofft_registry_entry *e = &mofft_registry[p->kernel];
+    const mofft_registry_entry *e = &ks->entries[p->kernel];
     p->stage_offset = count;
     size_t ns = 1, n_entries = 0;
     for (int k = 0; k < e->stage_count; ++k) {
@@ -383,23 +356,24 @@ static int split_better(const pass_split *a, const pass_split *b) {
   return 0;
 }
 
-static void search_splits(int precision_bits, size_t remaining,
-                          pass_split *cur, pass_split *best, int max_passes) {
+static void search_splits(const mofft_kernel_set *ks, int precision_bits,
+                          size_t remaining, pass_split *cur, pass_split *best,
+                          int max_passes) {
   if (remaining == 1) {
     if (cur->count > 0 && split_better(cur, best)) *best = *cur;
     return;
   }
   if (cur->count >= max_passes) return;
   if (best->count != 0 && cur->count + 1 > best->count) return;
-  for (int i = 0; i < mofft_registry_count; ++i) {
-    const mofft_registry_entry *e = &mofft_registry[i];
+  for (int i = 0; i < *ks->count; ++i) {
+    const mofft_registry_entry *e = &ks->entries[i];
     if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
       continue;
     const size_t l = (size_t)e->length;
     if (l < 2 || remaining % l) continue;
     int dup = 0;
     for (int j = 0; j < i; ++j) {
-      const mofft_registry_entry *f = &mofft_registry[j];
+      const mofft_registry_entry *f = &ks->entries[j];
       if (f->precision_bits == precision_bits && !f->rows &&
           f->length == e->length) { dup = 1; break; }
     }
@@ -407,19 +381,20 @@ static void search_splits(int precision_bits, size_t remaining,
     cur->lengths[cur->count] = l;
     cur->kernels[cur->count] = i;
     cur->count++;
-    search_splits(precision_bits, remaining / l, cur, best, max_passes);
+    search_splits(ks, precision_bits, remaining / l, cur, best, max_passes);
     cur->count--;
   }
 }
 
 /* Model plan: a single-pass kernel when one covers the length, otherwise the
  * pass decomposition with the fewest HBM round trips. */
-static mofft_status model_chain(size_t length, int precision_bits,
-                                int *radices, size_t *depth, uint64_t *mask) {
+static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
+                                int precision_bits, int *radices,
+                                size_t *depth, uint64_t *mask) {
   if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
-  const int single = find_kernel_by_length(precision_bits, 1, length);
+  const int single = find_kernel_by_length(ks, precision_bits, 1, length);
   if (single >= 0) {
-    const mofft_registry_entry *e = &mofft_registry[single];
+    const mofft_registry_entry *e = &ks->entries[single];
     for (int i = 0; i < e->stage_count; ++i)
       radices[i] = e->stages[e->stage_count - 1 - i];
     *depth = (size_t)e->stage_count;
@@ -430,13 +405,13 @@ static mofft_status model_chain(size_t length, int precision_bits,
   memset(&cur, 0, sizeof(cur));
   memset(&best, 0, sizeof(best));
   for (int max_passes = 2; max_passes <= 12 && best.count == 0; max_passes += 2)
-    search_splits(precision_bits, length, &cur, &best, max_passes);
+    search_splits(ks, precision_bits, length, &cur, &best, max_passes);
   if (best.count == 0) return MOFFT_UNSUPPORTED_SIZE;
   /* concatenate: the last pass is outermost */
   size_t d = 0;
   uint64_t m = 0;
   for (int p = best.count - 1; p >= 0; --p) {
-    const mofft_registry_entry *e = &mofft_registry[best.kernels[p]];
+    const mofft_registry_entry *e = &ks->entries[best.kernels[p]];
     if (d + (size_t)e->stage_count > MOFFT_MAX_PLAN_DEPTH)
       return MOFFT_UNSUPPORTED_SIZE;
     for (int i = e->stage_count - 1; i >= 0; --i) radices[d++] = e->stages[i];
@@ -528,13 +503,14 @@ static void plan_release(mofft_plan_common *p) {
   free(p);
 }
 
-static mofft_status fill_twiddles(mofft_plan_common *p) {
-  const mofft_schedule *s = &p->s;
+static mofft_status fill_schedule_twiddles(const mofft_schedule *s,
+                                           void **twiddles) {
   const size_t cb = complex_bytes(s->precision_bits);
+  *twiddles = NULL;
   if (s->twiddle_count == 0) return MOFFT_SUCCESS;
-  if (mofft_hip_malloc(&p->twiddles, sat_mul(s->twiddle_count, cb)) != 0)
+  if (mofft_hip_malloc(twiddles, sat_mul(s->twiddle_count, cb)) != 0)
     return MOFFT_ALLOCATION_FAILURE;
-  char *base = (char *)p->twiddles;
+  char *base = (char *)*twiddles;
   int rc = 0;
   if (s->pass_count > 1) {
     rc |= mofft_hip_fill_powers(base + s->tw_hi_offset * cb, s->precision_bits,
@@ -552,7 +528,7 @@ static mofft_status fill_twiddles(mofft_plan_common *p) {
   }
   for (int i = 0; i < s->pass_count && rc == 0; ++i) {
     const mofft_pass *ps = &s->passes[i];
-    const mofft_registry_entry *e = &mofft_registry[ps->kernel];
+    const mofft_registry_entry *e = &s->ks->entries[ps->kernel];
     size_t ns = 1, off = ps->stage_offset;
     for (int k = 0; k < e->stage_count; ++k) {
       const int r = e->stages[k];
@@ -565,7 +541,16 @@ static mofft_status fill_twiddles(mofft_plan_common *p) {
     }
   }
   if (rc == 0) rc = mofft_hip_synchronize(NULL);
-  return rc == 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
+  if (rc != 0) {
+    mofft_hip_free(*twiddles);
+    *twiddles = NULL;
+    return MOFFT_INTERNAL_ERROR;
+  }
+  return MOFFT_SUCCESS;
+}
+
+static mofft_status fill_twiddles(mofft_plan_common *p) {
+  return fill_schedule_twiddles(&p->s, &p->twiddles);
 }
 
 /* Build a plan from an explicit chain + mask (device resources included). */
@@ -578,8 +563,8 @@ static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
   if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
   mofft_plan_common *p = (mofft_plan_common *)calloc(1, sizeof(*p));
   if (!p) return MOFFT_ALLOCATION_FAILURE;
-  mofft_status st = build_schedule(length, batch, precision_bits, radices,
-                                   depth, mask, bank, &p->s);
+  mofft_status st = build_schedule(PASS_KERNELS, length, batch, precision_bits,
+                                   radices, depth, mask, bank, &p->s);
   if (st != MOFFT_SUCCESS) { free(p); return st; }
   p->direction = direction;
   p->placement = placement;
@@ -769,8 +754,8 @@ static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
   for (unsigned int bank = 0; bank < kernel_bank_count(); ++bank) {
     if (*count >= cap) return;
     mofft_schedule s;
-    if (build_schedule(length, batch, precision_bits, radices, depth, mask,
-                       bank, &s) != MOFFT_SUCCESS)
+    if (build_schedule(PASS_KERNELS, length, batch, precision_bits, radices,
+                       depth, mask, bank, &s) != MOFFT_SUCCESS)
       return;
     if (bank == 0) {
       base = s;
@@ -863,7 +848,7 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
   int radices[MOFFT_MAX_PLAN_DEPTH];
   size_t depth = 0;
   uint64_t mask = 0;
-  if (model_chain(length, precision_bits, radices, &depth, &mask) ==
+  if (model_chain(PASS_KERNELS, length, precision_bits, radices, &depth, &mask) ==
       MOFFT_SUCCESS)
     add_candidate(pool, &count, cap, length, batch, precision_bits, radices,
                   depth, mask);
@@ -1010,7 +995,7 @@ static mofft_status create_with_options(mofft_plan_common **out,
   if (w < 0) return MOFFT_WISDOM_NOT_FOUND;
   if (w == 0) {
     if (options.flags & MOFFT_PLAN_STRICT_WISDOM) return MOFFT_WISDOM_NOT_FOUND;
-    st = model_chain(length, precision_bits, radices, &depth, &mask);
+    st = model_chain(PASS_KERNELS, length, precision_bits, radices, &depth, &mask);
     if (st != MOFFT_SUCCESS) return st;
   }
   st = plan_from_chain(out, length, batch, precision_bits, direction,
@@ -1305,7 +1290,8 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
                                 &direct, &layout, &bank);
     if (w < 0) return MOFFT_WISDOM_NOT_FOUND;
     if (w == 0) {
-      mofft_status st = model_chain(length, precision_bits, chain, &d2, &mask);
+      mofft_status st = model_chain(PASS_KERNELS, length, precision_bits, chain,
+                                    &d2, &mask);
       if (st != MOFFT_SUCCESS) return st;
     }
     depth = d2;
@@ -1315,8 +1301,8 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
     memcpy(chain, radices, radix_count * sizeof(int));
   }
   mofft_schedule s;
-  mofft_status st = build_schedule(length, batch, precision_bits, chain, depth,
-                                   mask, 0, &s);
+  mofft_status st = build_schedule(PASS_KERNELS, length, batch, precision_bits,
+                                   chain, depth, mask, 0, &s);
   if (st != MOFFT_SUCCESS) return st;
   describe_schedule(&s, MOFFT_IN_PLACE, description);
   return MOFFT_SUCCESS;
@@ -1344,7 +1330,7 @@ mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
 static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
                             unsigned int bank) {
   mofft_schedule s;
-  mofft_status st = build_schedule(p->s.length, p->s.batch,
+  mofft_status st = build_schedule(PASS_KERNELS, p->s.length, p->s.batch,
                                    p->s.precision_bits, p->s.factors,
                                    (size_t)p->s.depth, mask, bank, &s);
   if (st != MOFFT_SUCCESS) return st;
@@ -1887,3 +1873,45 @@ const char *mofft_status_string(mofft_status status) {
 }
 
 const char *mofft_version(void) { return "0.4.0"; }
+
+/* ------------------------------------------------------------------ */
+/* internal entry points for the other runtime files (mofft_internal.h) */
+/* ------------------------------------------------------------------ */
+
+mofft_status mofft_internal_build_schedule(const mofft_kernel_set *ks,
+                                           size_t length, size_t batch,
+                                           int precision_bits,
+                                           const int *radices, size_t depth,
+                                           uint64_t mask, unsigned int bank,
+                                           mofft_schedule *s) {
+  return build_schedule(ks, length, batch, precision_bits, radices, depth,
+                        mask, bank, s);
+}
+
+mofft_status mofft_internal_model_chain(const mofft_kernel_set *ks,
+                                        size_t length, int precision_bits,
+                                        int *radices, size_t *depth,
+                                        uint64_t *mask) {
+  return model_chain(ks, length, precision_bits, radices, depth, mask);
+}
+
+int mofft_internal_wisdom_lookup(size_t length, int precision_bits,
+                                 int *radices, size_t *depth, uint64_t *mask) {
+  return wisdom_lookup(length, precision_bits, radices, depth, mask, NULL,
+                       NULL, NULL);
+}
+
+mofft_status mofft_internal_fill_twiddles(const mofft_schedule *s,
+                                          void **twiddles) {
+  return fill_schedule_twiddles(s, twiddles);
+}
+
+size_t mofft_internal_complex_bytes(int precision_bits) {
+  return complex_bytes(precision_bits);
+}
+
+void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
+                             mofft_plan_description *d) {
+  describe_schedule(s, MOFFT_IN_PLACE, d);
+  d->workspace_bytes = workspace_bytes;
+}

@@ -92,6 +92,73 @@ int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
 int mofft_hip_timer_begin(void *stream);
 int mofft_hip_timer_end(void *stream, double *seconds);
 
+/* ---- persistent uneven batch (one launch, device work queue) ---- */
+
+/* One transform of a batch whose whole DFT is a single row tile
+ * (gathered: adjacent columns of a tile belong to different jobs). */
+typedef struct {
+  const void *in;
+  void *out;
+  uint32_t flags;         /* MOFFT_PASS_SWAP_IN | MOFFT_PASS_SWAP_OUT */
+  uint32_t reserved;
+} mofft_batch_job;
+
+/* One pass description of the batch queue.  Three uses:
+ *   - a gathered row segment (jobs != NULL): column c of its tiles is the
+ *     whole transform jobs[c] (args.columns = job count, in/out unused);
+ *     one queue item per tile;
+ *   - a shared pass of a multi-pass transform (jobs == NULL, next < 0):
+ *     one queue item per tile, tiles of different workgroups; with
+ *     dep >= 0 a tile may start only once segment dep has finished
+ *     dep_target tiles, and signal != 0 makes its own tiles count;
+ *   - an owned transform (next >= 0 on its first pass): ONE queue item; the
+ *     workgroup that takes it runs every tile of this pass, then of segment
+ *     `next`, and so on, with a workgroup barrier between passes.
+ * Only queue segments (the first segment_count of the array) carry queue
+ * items; continuation passes of owned transforms follow them. */
+typedef struct {
+  mofft_pass_args args;
+  int64_t first_item;     /* queue index of this segment's first item */
+  const mofft_batch_job *jobs;
+  int32_t kind;           /* mofft_batch_registry index */
+  int32_t dep;
+  uint32_t dep_target;
+  uint32_t signal;
+  int32_t next;
+  uint32_t tiles;
+} mofft_batch_segment;
+
+/* Kernel argument of the persistent batch kernel.  counters[0] is the
+ * queue head, counters[1] the error flag (nonzero: a wait exceeded
+ * spin_limit polls and the launch gave up), counters[2 + s] the finished
+ * tiles of segment s (s < counter_count).  All of them are cleared by the
+ * launch. */
+typedef struct {
+  const mofft_batch_segment *segments;
+  unsigned int *counters;
+  int64_t segment_count;  /* queue segments */
+  int64_t total_items;
+  int64_t counter_count;
+  uint64_t spin_limit;
+} mofft_batch_header;
+
+extern const mofft_registry_entry mofft_batch_registry[];
+extern const int mofft_batch_registry_count;
+
+/* Resident workgroups per CU of the batch kernel with `lds_bytes` of
+ * dynamic LDS, and the CU count of the device. */
+int mofft_hip_batch_occupancy(int precision_bits, int lds_bytes,
+                              int *blocks_per_cu, int *cu_count);
+/* Clear the counters, then run the persistent kernel with `grid`
+ * workgroups (two kernel launches on `stream`, no memset node). */
+int mofft_hip_batch_launch(int precision_bits, const mofft_batch_header *h,
+                           int grid, int lds_bytes, void *stream);
+/* Capture mofft_hip_batch_launch into an instantiated HIP graph. */
+int mofft_hip_batch_graph(int precision_bits, const mofft_batch_header *h,
+                          int grid, int lds_bytes, void **graph_exec);
+int mofft_hip_graph_launch(void *graph_exec, void *stream);
+int mofft_hip_graph_destroy(void *graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,87 @@
+/*
+ * mofft_internal.h - host-side planning types shared by the runtime files
+ * (mofft.c: single plans; mofft_batch.c: persistent uneven batches).
+ * Not installed; nothing here is part of the public ABI.
+ */
+#ifndef MOFFT_INTERNAL_H
+#define MOFFT_INTERNAL_H
+
+#include "mofft_abi.h"
+#include "../../include/mofft.h"
+#include "../../include/mofft_hip.h"
+
+#include <stddef.h>
+#include <stdint.h>
+
+/* A family of compiled kernels a schedule may use: the pass-kernel
+ * registry (one launch per pass) or the persistent batch tile kinds. */
+typedef struct {
+  const mofft_registry_entry *entries;
+  const int *count;
+} mofft_kernel_set;
+
+extern const mofft_kernel_set mofft_pass_kernel_set;
+extern const mofft_kernel_set mofft_batch_kernel_set;
+
+typedef struct {
+  int kernel;            /* registry index */
+  size_t length;         /* points per column in LDS */
+  size_t stride;         /* n / length */
+  size_t span;           /* product of earlier pass lengths (ns) */
+  size_t stage_offset;   /* element offset of this pass's sub-stage table */
+  size_t stage_count;
+  size_t pass_offset;    /* element offset of the full pass twiddle table */
+  int pass_table;        /* 1: the full table exists for this pass */
+} mofft_pass;
+
+/* Passes after the first multiply by w_(ns*L)^((j%ns)*k).  Up to this many
+ * bytes per pass the factors come from a full table laid out so adjacent
+ * columns read adjacent entries (one coalesced load per point, L2/Infinity
+ * Cache resident); beyond it from a two-level table (two gathered loads and
+ * a complex product per point). */
+#define MOFFT_PASS_TABLE_BYTES ((size_t)8 << 20)
+
+typedef struct {
+  const mofft_kernel_set *ks;          /* kernel indices refer to this set */
+  size_t length;
+  size_t batch;
+  int precision_bits;
+  unsigned int bank;
+  int depth;
+  int factors[MOFFT_MAX_PLAN_DEPTH];   /* outer -> inner */
+  uint64_t pass_mask;
+  int pass_count;
+  mofft_pass passes[MOFFT_MAX_PLAN_DEPTH];
+  size_t chunk;
+  size_t workspace_bytes;
+  uint32_t split_shift;
+  size_t tw_hi_offset, tw_hi_count;
+  size_t tw_lo_offset, tw_lo_count;
+  size_t twiddle_count;                /* complex elements */
+} mofft_schedule;
+
+/* Chain + pass mask -> passes over kernels of `ks` (greedy longest match
+ * from the innermost stage; boundaries in `mask` always split). */
+mofft_status mofft_internal_build_schedule(const mofft_kernel_set *ks,
+                                           size_t length, size_t batch,
+                                           int precision_bits,
+                                           const int *radices, size_t depth,
+                                           uint64_t mask, unsigned int bank,
+                                           mofft_schedule *s);
+/* Analytical chain over kernels of `ks`: one pass when a row kernel covers
+ * the length, else the fewest passes (largest smallest pass). */
+mofft_status mofft_internal_model_chain(const mofft_kernel_set *ks,
+                                        size_t length, int precision_bits,
+                                        int *radices, size_t *depth,
+                                        uint64_t *mask);
+/* Compiled wisdom: 1 hit, 0 miss, -1 stale entry. */
+int mofft_internal_wisdom_lookup(size_t length, int precision_bits,
+                                 int *radices, size_t *depth, uint64_t *mask);
+/* Allocate the schedule's twiddle tables on the device and fill them. */
+mofft_status mofft_internal_fill_twiddles(const mofft_schedule *s,
+                                          void **twiddles);
+size_t mofft_internal_complex_bytes(int precision_bits);
+void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
+                             mofft_plan_description *d);
+
+#endif

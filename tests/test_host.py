@@ -243,3 +243,59 @@ def test_compiled_wisdom_matches_json():
         d = mf.describe(e["length"], 1, bits)
         assert d.chain == e["radices"], e
         assert d.pass_mask == e["blocked_transpose_mask"], e
+
+
+# ---- persistent uneven batch: host-side planning ----
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_batch_schedules(bits):
+    """Every power of two up to 4096 is one gathered row tile; up to 2^18
+    two strided passes of at most 512 points; longer ones three or more.
+    Every kind is a 256-thread tile of the batch registry."""
+    for k in range(1, 23):
+        n = 1 << k
+        d = mf.batch_describe_length(n, bits)
+        assert math.prod(d.chain) == n and math.prod(d.passes) == n
+        want = 1 if n <= 4096 else 2 if n <= 1 << 18 else 3
+        assert d.pass_count == want, (n, d.passes)
+        if d.pass_count > 1:
+            assert max(d.passes) <= 512
+        # workspace of an in-place transform: one transform when multi-pass
+        assert d.workspace_bytes == (0 if n <= 4096 else n * (8 if bits == 32 else 16))
+
+
+def test_batch_rejections_without_device():
+    L = mf.lib()
+    d = mf.PlanDescription()
+    for n in (0, 1, 3, 12, 1000):
+        assert L.mofft_batch_describe_length(32, n, ctypes.byref(d)) == mf.UNSUPPORTED_SIZE
+    assert L.mofft_batch_describe_length(16, 64, ctypes.byref(d)) == mf.INVALID_ARGUMENT
+    p = ctypes.c_void_p(1)
+    lengths = (ctypes.c_size_t * 2)(64, 128)
+    dirs = (ctypes.c_int * 2)(-1, 1)
+    ptrs = (ctypes.c_void_p * 2)(16, 32)
+    create = L.mofft_batch_plan_create
+    assert create(None, 32, 2, lengths, dirs, ptrs, ptrs, None) == mf.INVALID_ARGUMENT
+    assert create(ctypes.byref(p), 32, 0, lengths, dirs, ptrs, ptrs, None) == mf.INVALID_ARGUMENT
+    assert p.value is None
+    assert create(ctypes.byref(p), 48, 2, lengths, dirs, ptrs, ptrs, None) == mf.INVALID_ARGUMENT
+    bad_dirs = (ctypes.c_int * 2)(-1, 0)
+    assert create(ctypes.byref(p), 32, 2, lengths, bad_dirs, ptrs, ptrs, None) == mf.INVALID_ARGUMENT
+    odd = (ctypes.c_size_t * 2)(64, 96)
+    assert create(ctypes.byref(p), 32, 2, odd, dirs, ptrs, ptrs, None) == mf.UNSUPPORTED_SIZE
+    if not mf.device_available():
+        assert create(ctypes.byref(p), 32, 2, lengths, dirs, ptrs, ptrs, None) == mf.UNSUPPORTED_HARDWARE
+    assert L.mofft_batch_execute(None) == mf.INVALID_ARGUMENT
+    assert L.mofft_batch_check(None) == mf.INVALID_ARGUMENT
+    L.mofft_batch_plan_destroy(None)
+
+
+def test_batch_kinds_are_256_thread_tiles():
+    import json
+    man = json.loads((ROOT / "mofft_amd" / "csrc" / "generated" / "manifest.json").read_text())
+    kinds = man["batch_kinds"]
+    assert len(kinds) == 42
+    for k in kinds:
+        assert k["threads_per_column"] * k["columns_per_block"] == 256
+        assert math.prod(k["stages"]) == k["length"]
+        assert k["lds_bytes"] <= 64 * 1024
