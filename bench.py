@@ -49,6 +49,8 @@ def parse():
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-log2", default="6,18",
                     help="c4: lengths 2^U{a..b}")
+    ap.add_argument("--owned-tiles", type=int, default=0)
+    ap.add_argument("--item-tiles", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweep-max", type=int, default=20)
@@ -256,7 +258,8 @@ def run_batch(args):
     inplace = [bool(v) for v in rng.integers(0, 2, count)]
     ins = [torch.view_as_complex(torch.randn(n, 2, device=dev, generator=g)) for n in lengths]
     outs = [a if ip else torch.empty_like(a) for a, ip in zip(ins, inplace)]
-    bp = mf.BatchPlan(lengths, dirs, ins, outs, 32, graph=True)
+    bp = mf.BatchPlan(lengths, dirs, ins, outs, 32, graph=True,
+                      owned_tiles=args.owned_tiles, item_tiles=args.item_tiles)
     d = bp.description()
 
     def timed(fn, reps):
@@ -299,6 +302,7 @@ def run_batch(args):
                                "mixed placement, one persistent launch (hipGraph)",
                    "jobs": count, "distinct_lengths": d.distinct_lengths,
                    "multi_pass_jobs": d.multi_pass_jobs, "shared_jobs": d.shared_jobs,
+                   "owned_tiles": d.owned_tiles, "item_tiles": d.item_tiles,
                    "items": d.item_count,
                    "segments": d.segment_count, "grid": d.grid,
                    "blocks_per_cu": d.blocks_per_cu, "graph": bool(d.graph)},

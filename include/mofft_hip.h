@@ -137,18 +137,15 @@ int mofft_device_available(void);
  * the launch is captured once into a HIP graph that every execution
  * replays.  Lengths: powers of two >= 2 (one row tile up to 4096 points, two
  * strided passes up to 2^18, three beyond).  A multi-pass transform is run
- * whole by one workgroup unless it exceeds twice a fair share of the
- * launch; then its pass tiles spread over workgroups that synchronise
- * through device counters.  The radix chain of each length
+ * whole by one workgroup (owned) unless it is larger than `owned_tiles`;
+ * then its passes are split into items of `item_tiles` tiles spread over
+ * workgroups that synchronise through device counters (shared).  The radix chain of each length
  * is the compiled wisdom's when its passes exist as batch tile kinds, else
  * the analytical planner's over those kinds. */
 typedef struct mofft_batch_plan mofft_batch_plan;
 
 #define MOFFT_BATCH_NO_GRAPH 1u   /* launch directly, no captured graph */
-#define MOFFT_BATCH_SHARE_ALL 2u  /* every multi-pass transform spreads its
-                                     pass tiles over many workgroups (by
-                                     default only transforms above twice a
-                                     fair share of the launch do) */
+#define MOFFT_BATCH_SHARE_ALL 2u  /* every multi-pass transform is shared */
 
 typedef struct {
   uint32_t flags;        /* MOFFT_BATCH_* */
@@ -156,6 +153,10 @@ typedef struct {
   uint64_t spin_limit;   /* polls before a dependency wait gives up and
                             the execution reports MOFFT_INTERNAL_ERROR;
                             0 = default (about a second) */
+  uint32_t owned_tiles;  /* largest multi-pass transform (in tiles) run
+                            whole by one workgroup; 0 = automatic */
+  uint32_t item_tiles;   /* tiles per queue item of shared passes;
+                            0 = automatic */
 } mofft_batch_options;
 
 typedef struct {
@@ -163,6 +164,8 @@ typedef struct {
   size_t distinct_lengths;
   size_t multi_pass_jobs;
   size_t shared_jobs;         /* multi-pass transforms run as shared tiles */
+  size_t owned_tiles;         /* the owned/shared threshold in effect */
+  size_t item_tiles;          /* tiles per item of shared passes */
   size_t wisdom_lengths;      /* lengths whose chain came from wisdom */
   size_t segment_count;
   size_t item_count;          /* queue items of one execution */

@@ -119,12 +119,14 @@ class KernelInfo(ctypes.Structure):
 
 class BatchOptions(ctypes.Structure):
     _fields_ = [("flags", ctypes.c_uint32), ("grid", ctypes.c_uint32),
-                ("spin_limit", ctypes.c_uint64)]
+                ("spin_limit", ctypes.c_uint64), ("owned_tiles", ctypes.c_uint32),
+                ("item_tiles", ctypes.c_uint32)]
 
 
 class BatchDescription(ctypes.Structure):
     _fields_ = [("job_count", ctypes.c_size_t), ("distinct_lengths", ctypes.c_size_t),
                 ("multi_pass_jobs", ctypes.c_size_t), ("shared_jobs", ctypes.c_size_t),
+                ("owned_tiles", ctypes.c_size_t), ("item_tiles", ctypes.c_size_t),
                 ("wisdom_lengths", ctypes.c_size_t),
                 ("segment_count", ctypes.c_size_t), ("item_count", ctypes.c_size_t),
                 ("grid", ctypes.c_size_t), ("blocks_per_cu", ctypes.c_size_t),
@@ -406,7 +408,8 @@ class BatchPlan:
 
     def __init__(self, lengths: Sequence[int], directions: Sequence[int],
                  inputs, outputs, precision_bits: int = 32, graph: bool = True,
-                 grid: int = 0, spin_limit: int = 0, share_all: bool = False):
+                 grid: int = 0, spin_limit: int = 0, share_all: bool = False,
+                 owned_tiles: int = 0, item_tiles: int = 0):
         n = len(lengths)
         if not (len(directions) == len(inputs) == len(outputs) == n):
             raise ValueError("lengths, directions, inputs and outputs differ in size")
@@ -420,6 +423,8 @@ class BatchPlan:
         opts.flags = (0 if graph else BATCH_NO_GRAPH) | (BATCH_SHARE_ALL if share_all else 0)
         opts.grid = grid
         opts.spin_limit = spin_limit
+        opts.owned_tiles = owned_tiles
+        opts.item_tiles = item_tiles
         self._handle = ctypes.c_void_p()
         _check(lib().mofft_batch_plan_create(ctypes.byref(self._handle), precision_bits, n,
                                              self._lengths, self._dirs, ins, outs,

@@ -124,7 +124,7 @@ def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
             continue
         lines.append(f"      case {i}: batch_tile<{_targs(e)}>(sg, block, lds); return;")
     lines += ["      default: return;", "    }", "  }", "};", "",
-              f"__global__ void __launch_bounds__(kBatchThreads)",
+              f"__global__ void __launch_bounds__(kBatchThreads) MOFFT_BATCH_WAVES_F{precision}",
               f"batch_kernel_f{precision}(mofft_batch_header h) {{",
               f"  batch_loop<{t}, BatchDispatchF{precision}>(h);", "}", "",
               f"int batch_occupancy_f{precision}(int lds, int *bpc, int *cus) {{",

@@ -38,6 +38,21 @@
 
 #include "fft_pass.h"
 
+/* Build-time option: a minimum of waves per SIMD for the batch kernels
+ * (caps their registers; default: the compiler's choice). */
+#ifdef MOFFT_BATCH_MIN_WAVES_F32
+#define MOFFT_BATCH_WAVES_F32 \
+  __attribute__((amdgpu_waves_per_eu(MOFFT_BATCH_MIN_WAVES_F32)))
+#else
+#define MOFFT_BATCH_WAVES_F32
+#endif
+#ifdef MOFFT_BATCH_MIN_WAVES_F64
+#define MOFFT_BATCH_WAVES_F64 \
+  __attribute__((amdgpu_waves_per_eu(MOFFT_BATCH_MIN_WAVES_F64)))
+#else
+#define MOFFT_BATCH_WAVES_F64
+#endif
+
 namespace mofft {
 
 constexpr int kBatchThreads = 256;
@@ -146,15 +161,16 @@ load_segment(const mofft_batch_segment *p) {
 template <typename T, typename D>
 __device__ __forceinline__ void batch_loop(const mofft_batch_header &h) {
   extern __shared__ __align__(16) unsigned char mofft_lds[];
-  __shared__ int s_tile;
+  __shared__ int s_item;
   __shared__ int s_seg;
   T *lds = reinterpret_cast<T *>(mofft_lds);
   mofft_batch_segment c;
   int seg = -1;
-  int64_t block = 0;
-  bool owned = false;   /* inside an owned transform's chain of passes */
+  int64_t block = 0, end = 0;
+  bool active = false;  /* inside a run of tiles [block, end) of segment c */
+  bool owned = false;   /* ... that continues with the transform's next pass */
   for (;;) {
-    if (!owned) {
+    if (!active) {
       if (threadIdx.x == 0) {
         int sseg = -1;
         const int64_t t = (int64_t)__hip_atomic_fetch_add(
@@ -171,38 +187,45 @@ __device__ __forceinline__ void batch_loop(const mofft_batch_header &h) {
           if (dep >= 0 && !batch_wait(h, dep, h.segments[sseg].dep_target))
             sseg = -1;
         }
-        s_tile = (int)t;
+        s_item = (int)t;
         s_seg = sseg;
       }
       __syncthreads();
       /* uniform across the workgroup: keep them (and everything derived
        * from the segment) in scalar registers */
       seg = __builtin_amdgcn_readfirstlane(s_seg);
-      const int64_t tile = __builtin_amdgcn_readfirstlane(s_tile);
+      const int64_t item = __builtin_amdgcn_readfirstlane(s_item);
       if (seg < 0) break;
       c = load_segment(h.segments + seg);
       owned = c.next >= 0;
-      block = owned ? 0 : tile - c.first_item;
-      if (!owned && c.dep >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (owned) {
+        block = 0;
+        end = c.tiles;
+      } else {
+        block = (item - c.first_item) * (int64_t)c.group;
+        end = block + c.group < (int64_t)c.tiles ? block + c.group : c.tiles;
+        if (c.dep >= 0) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      }
+      active = true;
     }
     D::run(c.kind, c, block, lds);
-    const bool signal = !owned && c.signal;
+    const bool done = ++block == end;
+    const bool signal = done && !owned && c.signal;
     if (signal) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     /* LDS reuse by the next tile, and (owned) visibility of this pass's
      * stores to the whole workgroup before the next pass loads them */
     __syncthreads();
-    if (owned) {
-      if (++block == (int64_t)c.tiles) {
-        if (c.next < 0) {
-          owned = false;
-        } else {
-          c = load_segment(h.segments + c.next);
-          block = 0;
-        }
+    if (done) {
+      if (owned && c.next >= 0) {
+        c = load_segment(h.segments + c.next);
+        block = 0;
+        end = c.tiles;
+      } else {
+        if (signal && threadIdx.x == 0)
+          __hip_atomic_fetch_add(h.counters + 2 + seg, 1u, __ATOMIC_RELEASE,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+        active = false;
       }
-    } else if (signal && threadIdx.x == 0) {
-      __hip_atomic_fetch_add(h.counters + 2 + seg, 1u, __ATOMIC_RELEASE,
-                             __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

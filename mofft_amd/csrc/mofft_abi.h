@@ -108,9 +108,9 @@ typedef struct {
  *     whole transform jobs[c] (args.columns = job count, in/out unused);
  *     one queue item per tile;
  *   - a shared pass of a multi-pass transform (jobs == NULL, next < 0):
- *     one queue item per tile, tiles of different workgroups; with
- *     dep >= 0 a tile may start only once segment dep has finished
- *     dep_target tiles, and signal != 0 makes its own tiles count;
+ *     one queue item per `group` consecutive tiles, items run by different
+ *     workgroups; with dep >= 0 an item may start only once segment dep has
+ *     finished dep_target items, and signal != 0 makes its own items count;
  *   - an owned transform (next >= 0 on its first pass): ONE queue item; the
  *     workgroup that takes it runs every tile of this pass, then of segment
  *     `next`, and so on, with a workgroup barrier between passes.
@@ -126,12 +126,14 @@ typedef struct {
   uint32_t signal;
   int32_t next;
   uint32_t tiles;
+  uint32_t group;         /* tiles per queue item (gathered, shared) */
+  uint32_t reserved;
 } mofft_batch_segment;
 
 /* Kernel argument of the persistent batch kernel.  counters[0] is the
  * queue head, counters[1] the error flag (nonzero: a wait exceeded
  * spin_limit polls and the launch gave up), counters[2 + s] the finished
- * tiles of segment s (s < counter_count).  All of them are cleared by the
+ * items of segment s (s < counter_count).  All of them are cleared by the
  * launch. */
 typedef struct {
   const mofft_batch_segment *segments;
