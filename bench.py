@@ -45,7 +45,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "copy"])
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c3", "c4", "c5", "copy"])
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="c5: requests/s")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="c5: arrival window")
+    ap.add_argument("--serve-budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-streams", type=int, default=4)
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-log2", default="6,18",
                     help="c4: lengths 2^U{a..b}")
@@ -163,6 +167,8 @@ def run_gpu(args):
         return None, None
     elif args.workload == "c4":
         return run_batch(args)
+    elif args.workload == "c5":
+        return run_serve(args)
     else:
         return run_sweep(args)
 
@@ -315,6 +321,65 @@ def run_batch(args):
     bp.destroy()
     for p in plans.values():
         p.destroy()
+    print(json.dumps(line), flush=True)
+    return None, None
+
+
+def run_serve(args):
+    """Serving loop: Poisson arrivals, N = 2^U{8..24}, batch U{1..64}, 80%
+    FP32 / 20% FP64, inputs generated on the device from per-request seeds,
+    admission against a device-memory budget (mofft_server_run_trace)."""
+    import numpy as np
+    import mofft_amd as mf
+    rng = np.random.default_rng(2000)
+    count = int(args.serve_rate * args.serve_seconds)
+    ks = rng.integers(8, 25, count)
+    bs = rng.integers(1, 65, count)
+    prec = np.where(rng.random(count) < 0.8, 32, 64)
+    dirs = np.where(rng.random(count) < 0.5, mf.FORWARD, mf.BACKWARD)
+    seeds = rng.integers(1, 2**62, count)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, count))
+    reqs = [mf.Request(1 << int(k), int(b), int(p), int(d), None, int(sd), i)
+            for i, (k, b, p, d, sd) in enumerate(zip(ks, bs, prec, dirs, seeds))]
+    budget = int(args.serve_budget_gb * (1 << 30))
+    with mf.Server(budget, streams=args.serve_streams) as srv:
+        # resident plans for every (length, precision, direction) first
+        warm = [mf.Request(1 << k, 1, p, d, None, 1, 0)
+                for k in range(8, 25) for p in (32, 64) for d in (mf.FORWARD, mf.BACKWARD)]
+        srv.run_trace(warm, [0.0] * len(warm))
+        t0 = time.perf_counter()
+        cs = srv.run_trace(reqs, arrivals.tolist())
+        wall = time.perf_counter() - t0
+        st = srv.stats()
+    ok = [c for c in cs if c.status == mf.SUCCESS]
+    lat = [1e3 * (c.done_seconds - c.submit_seconds) for c in ok]
+    first = min(c.submit_seconds for c in ok)
+    last = max(c.done_seconds for c in ok)
+    flops = sum(5.0 * r.length * math.log2(r.length) * r.batch
+                for r, c in zip(reqs, cs) if c.status == mf.SUCCESS) - 0.0
+    span = last - first
+    alg = sum(2.0 * r.length * r.batch * (8 if r.precision_bits == 32 else 16)
+              for r, c in zip(reqs, cs) if c.status == mf.SUCCESS)
+    line = {
+        "metric": "served C2C FFT GFLOP/s (5N log2N); p50/p99 request latency",
+        "value": round(flops / span / 1e9, 2), "unit": "GFLOP/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "f32/f64 (80/20)",
+        "data": "synthetic (generated on device from per-request seeds)",
+        "config": {"workload": f"Poisson {args.serve_rate:g} req/s for {args.serve_seconds:g} s, "
+                               "N=2^U{8..24}, batch U{1..64}, 80% fp32 / 20% fp64",
+                   "requests": count, "budget_gb": args.serve_budget_gb,
+                   "streams": args.serve_streams},
+        "p50_ms": round(percentile(lat, 0.5), 3),
+        "p99_ms": round(percentile(lat, 0.99), 3),
+        "completed": len(ok), "rejected": count - len(ok),
+        "completed_per_s": round(len(ok) / span, 1),
+        "offered_per_s": args.serve_rate,
+        "hbm_gbs": round(alg / span / 1e9, 1),
+        "peak_bytes_in_use_gb": round(st.peak_bytes_in_use / 2**30, 2),
+        "peak_inflight": st.peak_inflight,
+        "resident_plans": st.resident_plans,
+        "wall_s": round(wall, 2),
+    }
     print(json.dumps(line), flush=True)
     return None, None
 

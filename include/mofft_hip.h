@@ -211,6 +211,97 @@ mofft_status mofft_batch_describe_length(int precision_bits, size_t length,
                                          mofft_plan_description *description);
 void mofft_batch_plan_destroy(mofft_batch_plan *plan);
 
+/* ---- serving loop ------------------------------------------------------
+ *
+ * No counterpart in the reference.  A server admits 1-D C2C requests of
+ * mixed length, batch and precision against a device-memory budget and
+ * runs them with continuous batching: requests that arrive while others
+ * are on the GPU are admitted (FIFO) as soon as an in-flight slot and
+ * budget free up, and run concurrently on several streams.  Plans (and
+ * their twiddles) stay resident per (length, precision, direction);
+ * request buffers come from one device arena of `budget_bytes` allocated
+ * at creation, so serving calls no device allocator.  Results are
+ * out-of-place in server-owned buffers that stay valid until released. */
+typedef struct mofft_server mofft_server;
+
+typedef struct {
+  size_t budget_bytes;    /* device arena for request buffers */
+  uint32_t streams;       /* device streams; 0 = 4 */
+  uint32_t max_inflight;  /* requests on the device at once; 0 = 4 * streams */
+  uint32_t max_batch;     /* largest batch of one request; 0 = 64 */
+  uint32_t flags;         /* reserved, 0 */
+} mofft_server_options;
+
+typedef struct {
+  size_t length;
+  size_t batch;
+  int precision_bits;
+  int direction;
+  const void *input;      /* device buffer of length * batch values, or
+                             NULL: generated on the device from `seed` by
+                             the counter-based generator of the library
+                             (mofft_server_generate) */
+  uint64_t seed;
+  uint64_t tag;           /* returned in the completion */
+} mofft_request;
+
+typedef struct {
+  uint64_t ticket;
+  uint64_t tag;
+  int status;             /* mofft_status of the request */
+  double submit_seconds;  /* host monotonic clock */
+  double start_seconds;   /* admitted and enqueued on a stream */
+  double done_seconds;    /* completion observed */
+  const void *output;     /* result (device), valid until released */
+  size_t bytes;
+} mofft_completion;
+
+typedef struct {
+  size_t submitted;
+  size_t completed;
+  size_t pending;
+  size_t inflight;
+  size_t resident_plans;
+  size_t bytes_in_use;
+  size_t peak_bytes_in_use;
+  size_t budget_bytes;
+  size_t peak_inflight;
+  double flops_completed;   /* 5 N log2 N per completed transform */
+  double bytes_completed;   /* 2 N sizeof(complex) per completed transform */
+} mofft_server_stats;
+
+void mofft_server_options_init(mofft_server_options *options);
+mofft_status mofft_server_create(mofft_server **server,
+                                 const mofft_server_options *options);
+/* Queue a request (never blocks).  MOFFT_ALLOCATION_FAILURE if it can never
+ * fit the budget, MOFFT_UNSUPPORTED_SIZE / MOFFT_INVALID_ARGUMENT as plan
+ * creation reports them. */
+mofft_status mofft_server_submit(mofft_server *server,
+                                 const mofft_request *request,
+                                 uint64_t *ticket);
+/* Retire finished work, admit pending requests, and return up to
+ * `capacity` new completions.  With wait != 0, block until at least one
+ * completion is available or nothing is pending or in flight. */
+mofft_status mofft_server_poll(mofft_server *server, mofft_completion *out,
+                               size_t capacity, size_t *count, int wait);
+/* Return a completed request's buffers to the arena. */
+mofft_status mofft_server_release(mofft_server *server, uint64_t ticket);
+mofft_status mofft_server_get_stats(const mofft_server *server,
+                                    mofft_server_stats *stats);
+/* Replay a trace in native code: request i is submitted `arrival_seconds[i]`
+ * after the start (wall clock); each result is released as soon as it is
+ * observed; completions[i] receives request i's record. */
+mofft_status mofft_server_run_trace(mofft_server *server,
+                                    const mofft_request *requests,
+                                    const double *arrival_seconds,
+                                    size_t count,
+                                    mofft_completion *completions);
+/* The server's input generator on a caller buffer (for checking results):
+ * `count` complex values from `seed`, as for requests with input == NULL. */
+mofft_status mofft_server_generate(void *out, int precision_bits,
+                                   size_t count, uint64_t seed, void *stream);
+void mofft_server_destroy(mofft_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -136,6 +136,35 @@ class BatchDescription(ctypes.Structure):
                 ("graph", ctypes.c_int)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("budget_bytes", ctypes.c_size_t), ("streams", ctypes.c_uint32),
+                ("max_inflight", ctypes.c_uint32), ("max_batch", ctypes.c_uint32),
+                ("flags", ctypes.c_uint32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_size_t), ("batch", ctypes.c_size_t),
+                ("precision_bits", ctypes.c_int), ("direction", ctypes.c_int),
+                ("input", ctypes.c_void_p), ("seed", ctypes.c_uint64),
+                ("tag", ctypes.c_uint64)]
+
+
+class Completion(ctypes.Structure):
+    _fields_ = [("ticket", ctypes.c_uint64), ("tag", ctypes.c_uint64),
+                ("status", ctypes.c_int), ("submit_seconds", ctypes.c_double),
+                ("start_seconds", ctypes.c_double), ("done_seconds", ctypes.c_double),
+                ("output", ctypes.c_void_p), ("bytes", ctypes.c_size_t)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_size_t), ("completed", ctypes.c_size_t),
+                ("pending", ctypes.c_size_t), ("inflight", ctypes.c_size_t),
+                ("resident_plans", ctypes.c_size_t), ("bytes_in_use", ctypes.c_size_t),
+                ("peak_bytes_in_use", ctypes.c_size_t), ("budget_bytes", ctypes.c_size_t),
+                ("peak_inflight", ctypes.c_size_t), ("flops_completed", ctypes.c_double),
+                ("bytes_completed", ctypes.c_double)]
+
+
 BATCH_NO_GRAPH = 1
 BATCH_SHARE_ALL = 2
 
@@ -210,6 +239,17 @@ _SIGS += [
     ("mofft_batch_describe", _I, [_P, ctypes.POINTER(BatchDescription)]),
     ("mofft_batch_describe_length", _I, [_I, _SZ, ctypes.POINTER(PlanDescription)]),
     ("mofft_batch_plan_destroy", None, [_P]),
+    ("mofft_server_options_init", None, [ctypes.POINTER(ServerOptions)]),
+    ("mofft_server_create", _I, [ctypes.POINTER(_P), ctypes.POINTER(ServerOptions)]),
+    ("mofft_server_submit", _I, [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)]),
+    ("mofft_server_poll", _I, [_P, ctypes.POINTER(Completion), _SZ, ctypes.POINTER(_SZ), _I]),
+    ("mofft_server_release", _I, [_P, ctypes.c_uint64]),
+    ("mofft_server_get_stats", _I, [_P, ctypes.POINTER(ServerStats)]),
+    ("mofft_server_run_trace", _I,
+     [_P, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_double), _SZ,
+      ctypes.POINTER(Completion)]),
+    ("mofft_server_generate", _I, [_P, _I, _SZ, ctypes.c_uint64, _P]),
+    ("mofft_server_destroy", None, [_P]),
 ]
 
 
@@ -472,6 +512,79 @@ class BatchPlan:
 
     def __exit__(self, *exc):
         self.destroy()
+
+
+class Server:
+    """Continuous-batching request loop (mofft_server_*): admission against
+    a device-memory budget, resident plans, results in server buffers."""
+
+    def __init__(self, budget_bytes: int, streams: int = 0, max_inflight: int = 0,
+                 max_batch: int = 0):
+        o = ServerOptions()
+        lib().mofft_server_options_init(ctypes.byref(o))
+        o.budget_bytes = int(budget_bytes)
+        o.streams = streams
+        o.max_inflight = max_inflight
+        o.max_batch = max_batch
+        self._h = ctypes.c_void_p()
+        _check(lib().mofft_server_create(ctypes.byref(self._h), ctypes.byref(o)),
+               "server_create")
+
+    def submit(self, length: int, batch: int = 1, precision_bits: int = 32,
+               direction: int = FORWARD, seed: int = 0, input=None, tag: int = 0) -> int:
+        r = Request(length, batch, precision_bits, direction,
+                    None if input is None else (input if isinstance(input, int) else _ptr(input)),
+                    seed, tag)
+        t = ctypes.c_uint64()
+        _check(lib().mofft_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)),
+               f"server_submit N={length} batch={batch}")
+        return t.value
+
+    def poll(self, wait: bool = False, capacity: int = 64) -> list:
+        buf = (Completion * capacity)()
+        n = ctypes.c_size_t()
+        _check(lib().mofft_server_poll(self._h, buf, capacity, ctypes.byref(n), int(wait)),
+               "server_poll")
+        return [buf[i] for i in range(n.value)]
+
+    def release(self, ticket: int) -> None:
+        _check(lib().mofft_server_release(self._h, ticket), "server_release")
+
+    def stats(self) -> ServerStats:
+        st = ServerStats()
+        _check(lib().mofft_server_get_stats(self._h, ctypes.byref(st)), "server_stats")
+        return st
+
+    def run_trace(self, requests: Sequence[Request], arrivals: Sequence[float]):
+        n = len(requests)
+        reqs = (Request * n)(*requests)
+        arr = (ctypes.c_double * n)(*arrivals)
+        out = (Completion * n)()
+        _check(lib().mofft_server_run_trace(self._h, reqs, arr, n, out), "server_run_trace")
+        return out
+
+    def destroy(self) -> None:
+        if self._h and self._h.value:
+            lib().mofft_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+def server_generate(out, precision_bits: int, count: int, seed: int, stream: int = 0) -> None:
+    """The server's seeded device input generator on a caller buffer."""
+    _check(lib().mofft_server_generate(_ptr(out), precision_bits, count, seed, stream or None),
+           "server_generate")
 
 
 def estimated_plan_breakdown(length: int, precision_bits: int, radices,

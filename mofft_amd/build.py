@@ -49,7 +49,8 @@ def _generate() -> None:
 
 
 def _sources() -> tuple[list[Path], list[Path], list[Path]]:
-    c_srcs = [CSRC / "mofft.c", CSRC / "mofft_batch.c", GEN / "mofft_registry.c",
+    c_srcs = [CSRC / "mofft.c", CSRC / "mofft_batch.c", CSRC / "mofft_server.cpp",
+              GEN / "mofft_registry.c",
               GEN / "mofft_wisdom.c",
               GEN / "mofft_launch_table.cpp"]
     hip_srcs = [CSRC / "hip" / "device.hip"]

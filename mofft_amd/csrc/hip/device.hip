@@ -79,6 +79,22 @@ __global__ void fill_pass_kernel(T *out, int64_t ns, int64_t L) {
   out[2 * i] = (T)c;
   out[2 * i + 1] = (T)s;
 }
+__device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void fill_uniform_kernel(T *out, int64_t values, uint64_t seed) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < values;
+       i += stride) {
+    const uint64_t z = splitmix64(seed + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull);
+    const double u = (double)(z >> 11) * 0x1.0p-53;
+    out[i] = (T)(2.0 * u - 1.0);
+  }
+}
 }  // namespace mofft
 
 extern "C" {
@@ -266,6 +282,62 @@ int mofft_hip_graph_launch(void *graph_exec, void *stream) {
 int mofft_hip_graph_destroy(void *graph_exec) {
   if (!graph_exec) return 0;
   return (int)hipGraphExecDestroy((hipGraphExec_t)graph_exec);
+}
+
+int mofft_hip_stream_create(void **stream) {
+  if (!stream) return (int)hipErrorInvalidValue;
+  hipStream_t s = nullptr;
+  const hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  *stream = e == hipSuccess ? (void *)s : nullptr;
+  return (int)e;
+}
+
+int mofft_hip_stream_destroy(void *stream) {
+  if (!stream) return 0;
+  return (int)hipStreamDestroy((hipStream_t)stream);
+}
+
+int mofft_hip_event_create(void **event) {
+  if (!event) return (int)hipErrorInvalidValue;
+  hipEvent_t ev = nullptr;
+  const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  *event = e == hipSuccess ? (void *)ev : nullptr;
+  return (int)e;
+}
+
+int mofft_hip_event_destroy(void *event) {
+  if (!event) return 0;
+  return (int)hipEventDestroy((hipEvent_t)event);
+}
+
+int mofft_hip_event_record(void *event, void *stream) {
+  return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
+}
+
+int mofft_hip_event_query(void *event) {
+  const hipError_t e = hipEventQuery((hipEvent_t)event);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return -(int)e;
+}
+
+int mofft_hip_event_synchronize(void *event) {
+  return (int)hipEventSynchronize((hipEvent_t)event);
+}
+
+int mofft_hip_fill_uniform(void *out, int precision_bits, int64_t count,
+                           uint64_t seed, void *stream) {
+  if (count <= 0) return 0;
+  const int64_t values = 2 * count;
+  int64_t blocks = (values + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  if (precision_bits == 32)
+    hipLaunchKernelGGL(mofft::fill_uniform_kernel<float>, dim3((unsigned)blocks),
+                       dim3(256), 0, (hipStream_t)stream, (float *)out, values, seed);
+  else
+    hipLaunchKernelGGL(mofft::fill_uniform_kernel<double>, dim3((unsigned)blocks),
+                       dim3(256), 0, (hipStream_t)stream, (double *)out, values, seed);
+  return (int)hipGetLastError();
 }
 
 }  // extern "C"

@@ -586,16 +586,17 @@ static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
 /* execution                                                            */
 /* ------------------------------------------------------------------ */
 
-static int launch_schedule(const mofft_plan_common *p, const void *in,
-                           void *out, void *work, void *stream) {
+static int launch_schedule_count(const mofft_plan_common *p, const void *in,
+                                 void *out, void *work, void *stream,
+                                 size_t count) {
   const mofft_schedule *s = &p->s;
   const size_t cb = complex_bytes(s->precision_bits);
   const char *tw = (const char *)p->twiddles;
   const int backward = p->direction == MOFFT_BACKWARD;
   const int P = s->pass_count;
   const int in_place = in == out;
-  for (size_t c0 = 0; c0 < s->batch; c0 += s->chunk) {
-    const size_t cnt = s->batch - c0 < s->chunk ? s->batch - c0 : s->chunk;
+  for (size_t c0 = 0; c0 < count; c0 += s->chunk) {
+    const size_t cnt = count - c0 < s->chunk ? count - c0 : s->chunk;
     const char *src_chunk = (const char *)in + c0 * s->length * cb;
     char *dst_chunk = (char *)out + c0 * s->length * cb;
     const void *src = src_chunk;
@@ -634,6 +635,11 @@ static int launch_schedule(const mofft_plan_common *p, const void *in,
     }
   }
   return 0;
+}
+
+static int launch_schedule(const mofft_plan_common *p, const void *in,
+                           void *out, void *work, void *stream) {
+  return launch_schedule_count(p, in, out, work, stream, p->s.batch);
 }
 
 static mofft_status execute_device(const mofft_plan_common *p, const void *in,
@@ -1914,4 +1920,27 @@ void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
                              mofft_plan_description *d) {
   describe_schedule(s, MOFFT_IN_PLACE, d);
   d->workspace_bytes = workspace_bytes;
+}
+
+size_t mofft_internal_workspace_count(const void *plan, size_t count) {
+  const mofft_plan_common *p = (const mofft_plan_common *)plan;
+  if (!p || !schedule_workspace(&p->s, p->placement)) return 0;
+  const size_t c = count < p->s.chunk ? count : p->s.chunk;
+  return sat_mul(sat_mul(c, p->s.length), complex_bytes(p->s.precision_bits));
+}
+
+mofft_status mofft_internal_execute_count(const void *plan, const void *in,
+                                          void *out, void *work, size_t count,
+                                          void *stream) {
+  const mofft_plan_common *p = (const mofft_plan_common *)plan;
+  if (!p || !in || !out || count == 0 || count > p->s.batch)
+    return MOFFT_INVALID_ARGUMENT;
+  if ((p->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (!work) work = p->workspace;
+  if (mofft_internal_workspace_count(plan, count) && !work)
+    return MOFFT_INVALID_ARGUMENT;
+  return launch_schedule_count(p, in, out, work, stream, count) == 0
+             ? MOFFT_SUCCESS
+             : MOFFT_INTERNAL_ERROR;
 }

@@ -161,6 +161,24 @@ int mofft_hip_batch_graph(int precision_bits, const mofft_batch_header *h,
 int mofft_hip_graph_launch(void *graph_exec, void *stream);
 int mofft_hip_graph_destroy(void *graph_exec);
 
+/* ---- streams, events and device-side inputs (serving loop) ---- */
+
+/* Streams are created with the default priority. */
+int mofft_hip_stream_create(void **stream);
+int mofft_hip_stream_destroy(void *stream);
+int mofft_hip_event_create(void **event);
+int mofft_hip_event_destroy(void *event);
+int mofft_hip_event_record(void *event, void *stream);
+/* 1 when the event has completed, 0 when not yet, negative on error. */
+int mofft_hip_event_query(void *event);
+int mofft_hip_event_synchronize(void *event);
+/* Interleaved complex values from a seed: component c (0 = real,
+ * 1 = imaginary) of element i is 2u - 1 with u the top 53 bits of
+ * splitmix64(seed + (2i + c + 1) * 0x9E3779B97F4A7C15) times 2^-53, rounded
+ * to the precision.  count = elements. */
+int mofft_hip_fill_uniform(void *out, int precision_bits, int64_t count,
+                           uint64_t seed, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,4 +84,13 @@ size_t mofft_internal_complex_bytes(int precision_bits);
 void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
                              mofft_plan_description *d);
 
+/* Enqueue the first `count` (<= the plan's batch) transforms of a batched
+ * plan (a mofft_plan_f32 * or mofft_plan_f64 *) on `stream`, device
+ * buffers; `work` (NULL: the plan's own) holds
+ * mofft_internal_workspace_count(plan, count) bytes. */
+size_t mofft_internal_workspace_count(const void *plan, size_t count);
+mofft_status mofft_internal_execute_count(const void *plan, const void *in,
+                                          void *out, void *work, size_t count,
+                                          void *stream);
+
 #endif

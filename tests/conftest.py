@@ -56,3 +56,25 @@ def gpu(mofft):
     if not mofft.device_available():
         pytest.fail("gpu test ran without a usable gfx950 device")
     return mofft
+
+
+_SM_GAMMA = np.uint64(0x9E3779B97F4A7C15)
+
+
+def seeded_device_uniform(seed, count, bits):
+    """Host restatement of the library's device input generator
+    (mofft_hip_fill_uniform / mofft_server_generate): component c of complex
+    element i is 2u - 1, u = top 53 bits of
+    splitmix64(seed + (2i + c + 1) * 0x9E3779B97F4A7C15) * 2^-53."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, 2 * count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _SM_GAMMA
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    u = (z >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    v = 2.0 * u - 1.0
+    if bits == 32:
+        v = v.astype(np.float32)
+        return v.view(np.complex64)
+    return v.view(np.complex128)

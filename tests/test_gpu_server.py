@@ -1,0 +1,148 @@
+"""GPU tests of the serving loop (mofft_server_* in mofft_hip.h).
+
+Requests of mixed length, batch, precision and direction are admitted
+against a device-memory budget and run concurrently on several streams.
+The central check: every result is the same alone (a server that runs one
+request at a time) and in the mix (many requests admitted while others run,
+budget binding) - bit-identical, which is within any tolerance - and each
+is within the oracle bar (FP32 1e-6*log2N, FP64 1e-15*log2N relative L2).
+Inputs are generated on the device from seeds; the generator's host
+restatement (conftest.seeded_device_uniform) regenerates them for the
+oracle.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import seeded_device_uniform
+
+pytestmark = pytest.mark.gpu
+
+
+def tol(n, bits):
+    return (1e-6 if bits == 32 else 1e-15) * max(1.0, math.log2(n))
+
+
+@pytest.fixture(scope="module")
+def hip_copy():
+    import torch
+    assert torch.cuda.is_available()
+    rt = ctypes.CDLL("libamdhip64.so")
+    rt.hipMemcpy.restype = ctypes.c_int
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    rt.hipDeviceSynchronize.restype = ctypes.c_int
+
+    def copy(ptr, n, batch, bits):
+        out = np.empty((batch, n), np.complex64 if bits == 32 else np.complex128)
+        assert rt.hipDeviceSynchronize() == 0
+        assert rt.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0   # device to host
+        return out
+    return copy
+
+
+def mixed_requests(seed, count, kmin, kmax, bmax):
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = 1 << int(rng.integers(kmin, kmax + 1))
+        b = int(rng.integers(1, bmax + 1))
+        bits = 32 if rng.random() < 0.8 else 64
+        d = -1 if rng.random() < 0.5 else 1
+        reqs.append((n, b, bits, d, int(rng.integers(1, 2**62)), i))
+    return reqs
+
+
+def test_generator_matches_restatement(gpu, hip_copy):
+    import torch
+    for bits in (32, 64):
+        n = 5000
+        buf = torch.empty(2 * n, dtype=torch.float32 if bits == 32 else torch.float64, device="cuda")
+        gpu.server_generate(buf, bits, n, 12345)
+        torch.cuda.synchronize()
+        got = buf.cpu().numpy().view(np.complex64 if bits == 32 else np.complex128)
+        want = seeded_device_uniform(12345, n, bits)
+        assert np.array_equal(got, want)
+
+
+def test_same_alone_and_in_the_mix(gpu, hip_copy):
+    reqs = mixed_requests(77, 40, 8, 17, 6)
+    alone = {}
+    with gpu.Server(1 << 30, streams=1, max_inflight=1) as srv:
+        for (n, b, bits, d, seed, tag) in reqs:
+            t = srv.submit(n, b, bits, d, seed=seed, tag=tag)
+            (c,) = srv.poll(wait=True)
+            assert c.ticket == t and c.status == gpu.SUCCESS
+            alone[tag] = hip_copy(c.output, n, b, bits)
+            srv.release(t)
+        assert srv.stats().peak_inflight == 1
+    budget = 24 << 20        # binding: requests wait for each other's memory
+    mix = {}
+    with gpu.Server(budget, streams=4, max_inflight=16) as srv:
+        for (n, b, bits, d, seed, tag) in reqs:
+            srv.submit(n, b, bits, d, seed=seed, tag=tag)
+        while len(mix) < len(reqs):
+            for c in srv.poll(wait=True):
+                assert c.status == gpu.SUCCESS
+                n, b, bits = reqs[c.tag][:3]
+                mix[c.tag] = hip_copy(c.output, n, b, bits)
+                srv.release(c.ticket)
+        st = srv.stats()
+        assert st.completed == len(reqs) and st.pending == 0 and st.inflight == 0
+        assert st.peak_bytes_in_use <= budget
+        assert st.peak_inflight > 1           # work overlapped on the device
+        assert st.bytes_in_use == 0
+    for (n, b, bits, d, seed, tag) in reqs:
+        assert np.array_equal(alone[tag], mix[tag]), tag
+        x = seeded_device_uniform(seed, n * b, bits).reshape(b, n).astype(np.complex128)
+        ref = oracle.dft(x, d)
+        assert oracle.rel_l2(mix[tag], ref) <= tol(n, bits), (tag, n, b, bits)
+
+
+def test_budget_and_argument_checks(gpu):
+    with gpu.Server(1 << 20) as srv:
+        with pytest.raises(gpu.MofftError) as e:
+            srv.submit(1 << 16, 4, 64)         # 2 x 4 MiB > 1 MiB budget
+        assert e.value.status == gpu.ALLOCATION_FAILURE
+        with pytest.raises(gpu.MofftError) as e:
+            srv.submit(1 << 10, 65, 32)        # above max_batch
+        assert e.value.status == gpu.INVALID_ARGUMENT
+        with pytest.raises(gpu.MofftError) as e:
+            srv.submit(17, 1, 32)
+        assert e.value.status == gpu.UNSUPPORTED_SIZE
+        with pytest.raises(gpu.MofftError):
+            srv.release(12345)
+
+
+def test_caller_input_matches_plan(gpu, hip_copy):
+    import torch
+    n, b = 4096, 3
+    x = (np.random.default_rng(4).standard_normal((b, n, 2))).astype(np.float64)
+    d_in = torch.from_numpy(x).to("cuda")
+    with gpu.Server(64 << 20) as srv:
+        t = srv.submit(n, b, 64, gpu.BACKWARD, input=d_in)
+        (c,) = srv.poll(wait=True)
+        got = hip_copy(c.output, n, b, 64)
+        srv.release(t)
+    xc = x.view(np.complex128).reshape(b, n)
+    y = np.empty_like(xc)
+    with gpu.Plan(n, b, 64, gpu.BACKWARD) as p:
+        p.execute(xc, y)
+    assert np.array_equal(got, y)
+
+
+def test_poisson_trace(gpu):
+    reqs = mixed_requests(5, 300, 8, 20, 16)
+    rng = np.random.default_rng(6)
+    arrivals = np.cumsum(rng.exponential(1 / 2000.0, len(reqs)))
+    with gpu.Server(2 << 30, streams=4, max_inflight=16) as srv:
+        cs = srv.run_trace([gpu.Request(n, b, bits, d, None, seed, tag)
+                            for (n, b, bits, d, seed, tag) in reqs], arrivals.tolist())
+        st = srv.stats()
+    assert st.completed == len(reqs) and st.bytes_in_use == 0
+    lat = [c.done_seconds - c.submit_seconds for c in cs]
+    assert all(c.status == gpu.SUCCESS for c in cs)
+    assert all(l > 0 for l in lat)
+    assert sorted(c.tag for c in cs) == list(range(len(reqs)))

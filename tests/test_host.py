@@ -299,3 +299,43 @@ def test_batch_kinds_are_256_thread_tiles():
         assert k["threads_per_column"] * k["columns_per_block"] == 256
         assert math.prod(k["stages"]) == k["length"]
         assert k["lds_bytes"] <= 64 * 1024
+
+
+def test_server_without_device_and_options():
+    L = mf.lib()
+    s = ctypes.c_void_p(1)
+    o = mf.ServerOptions()
+    L.mofft_server_options_init(ctypes.byref(o))
+    assert o.budget_bytes == 1 << 30 and o.flags == 0
+    assert L.mofft_server_create(None, ctypes.byref(o)) == mf.INVALID_ARGUMENT
+    bad = mf.ServerOptions(16, 0, 0, 0, 0)
+    assert L.mofft_server_create(ctypes.byref(s), ctypes.byref(bad)) == mf.INVALID_ARGUMENT
+    assert s.value is None
+    if not mf.device_available():
+        assert L.mofft_server_create(ctypes.byref(s), ctypes.byref(o)) == mf.UNSUPPORTED_HARDWARE
+    L.mofft_server_destroy(None)
+    assert L.mofft_server_poll(None, None, 0, None, 0) == mf.INVALID_ARGUMENT
+
+
+def test_device_generator_restatement_properties():
+    """The host restatement of the seeded device generator: values in
+    [-1, 1), deterministic per seed, independent across seeds, FP32 the
+    rounding of FP64 (the GPU tests pin it bit-exactly to the device)."""
+    import numpy as np
+    from conftest import seeded_device_uniform
+    a = seeded_device_uniform(1, 100000, 64)
+    assert np.array_equal(a, seeded_device_uniform(1, 100000, 64))
+    v = a.view(np.float64)
+    assert v.min() >= -1.0 and v.max() < 1.0
+    assert abs(v.mean()) < 0.01 and abs(v.var() - 1 / 3) < 0.01
+    b = seeded_device_uniform(2, 100000, 64).view(np.float64)
+    assert abs(np.corrcoef(v, b)[0, 1]) < 0.01
+    f = seeded_device_uniform(1, 1000, 32).view(np.float32)
+    assert np.array_equal(f, v[:2000].astype(np.float32))
+    # first value by hand: splitmix64(1 + gamma)
+    m = (1 << 64) - 1
+    z = (1 + 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z ^= z >> 31
+    assert v[0] == 2.0 * ((z >> 11) * 2.0 ** -53) - 1.0
