@@ -385,7 +385,10 @@ def run_serve(args):
 
 
 def run_sweep(args):
-    """FP32 forward out-of-place, N = 2^k, batch = 2^24 / N (one line per N)."""
+    """FP32 forward out-of-place, N = 2^k, batch = 2^24 / N: the plain FP32
+    form and the matrix-core split-precision form (one line per N and
+    form).  `rt_err` is the forward->backward round-trip relative L2 error
+    measured on the device (the oracle errors per N come from the tests)."""
     import torch
     import mofft_amd as mf
     dev = torch.device("cuda:0")
@@ -395,29 +398,37 @@ def run_sweep(args):
     total = 1 << 24
     x = torch.view_as_complex(torch.rand(total, 2, device=dev, generator=g) * 2 - 1)
     y = torch.empty_like(x)
+    z = torch.empty_like(x)
     for k in range(4, args.sweep_max + 1):
         n = 1 << k
         batch = total // n
-        p = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False)
-        for _ in range(3):
-            p.execute_async(x, y, sh)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        reps = 20
-        e0.record()
-        for _ in range(reps):
-            p.execute_async(x, y, sh)
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        gf = 5.0 * n * k * batch / (ms * 1e-3) / 1e9
-        gbs = 2.0 * total * 8 / (ms * 1e-3) / 1e9
-        print(json.dumps({"workload": "fp32 forward out-of-place sweep", "length": n,
-                          "batch": batch, "passes": p.description().passes,
-                          "ms": round(ms, 4), "gflops": round(gf, 1),
-                          "hbm_gbs": round(gbs, 1),
-                          "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}), flush=True)
-        p.destroy()
+        for form in ("plain", "mfma"):
+            p = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False, mfma=form == "mfma")
+            q = mf.Plan(n, batch, 32, mf.BACKWARD, in_place=False, mfma=form == "mfma")
+            for _ in range(3):
+                p.execute_async(x, y, sh)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 20
+            e0.record()
+            for _ in range(reps):
+                p.execute_async(x, y, sh)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            q.execute_async(y, z, sh)
+            torch.cuda.synchronize()
+            rt = (torch.linalg.vector_norm(z / n - x) / torch.linalg.vector_norm(x)).item()
+            gf = 5.0 * n * k * batch / (ms * 1e-3) / 1e9
+            gbs = 2.0 * total * 8 / (ms * 1e-3) / 1e9
+            print(json.dumps({"workload": "fp32 forward out-of-place sweep", "form": form,
+                              "length": n, "batch": batch, "passes": p.description().passes,
+                              "ms": round(ms, 4), "gflops": round(gf, 1),
+                              "hbm_gbs": round(gbs, 1),
+                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                              "rt_err": float(f"{rt:.3e}")}), flush=True)
+            p.destroy()
+            q.destroy()
     return None, None
 
 

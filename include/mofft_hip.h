@@ -25,7 +25,12 @@ extern "C" {
 
 #define MOFFT_MAX_PASSES 16
 
-/* Batched plan creation.  `batch` >= 1; batch == 1 is the mofft.h plan. */
+/* Batched plan creation.  `batch` >= 1; batch == 1 is the mofft.h plan.
+ * options->flags may add MOFFT_PLAN_MFMA (FP32, PRETUNED, no other flag):
+ * the radix-16 sub-stages run on the matrix cores as split-precision FP16
+ * products accumulated in FP32 (kernel bank 3; error stated per length by
+ * the tests and the benchmark), with the analytical radix chain. */
+#define MOFFT_PLAN_MFMA 0x100u
 mofft_status mofft_plan_create_batched_f32(
     mofft_plan_f32 **plan, size_t length, size_t batch,
     mofft_direction direction, mofft_placement placement,

@@ -38,6 +38,8 @@ STRICT_WISDOM = 1
 FLAG_MEASURE = 1
 FLAG_WISDOM_ONLY = 2
 MAX_PLAN_DEPTH = 64
+PLAN_MFMA = 0x100          # mofft_hip.h: radix-16 stages on the matrix cores
+MFMA_BANK = 3
 MAX_PASSES = 16
 
 
@@ -333,7 +335,8 @@ class Plan:
                  direction: int = FORWARD, in_place: bool = False,
                  radices: Optional[Sequence[int]] = None,
                  mode: int = PRETUNED, flags: int = 0, candidate_limit: int = 0,
-                 timing_samples: int = 0, pass_mask: Optional[int] = None):
+                 timing_samples: int = 0, pass_mask: Optional[int] = None,
+                 mfma: bool = False):
         if precision_bits not in (32, 64):
             raise ValueError("precision_bits must be 32 or 64")
         self.length = int(length)
@@ -350,7 +353,7 @@ class Plan:
             opts = PlanOptions()
             L.mofft_plan_options_init(ctypes.byref(opts))
             opts.mode = mode
-            opts.flags = flags
+            opts.flags = flags | (PLAN_MFMA if mfma else 0)
             opts.candidate_limit = candidate_limit
             opts.timing_samples = timing_samples
             st = getattr(L, f"mofft_plan_create_batched_{self._s}")(

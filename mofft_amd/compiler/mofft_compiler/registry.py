@@ -144,12 +144,31 @@ def _make(precision: int, length: int, rows: int,
     return e
 
 
+MFMA_VARIANT = 3
+
+
+def mfma_fits(e: Entry) -> bool:
+    """Tile shape constraints of the matrix-core radix-16 sub-stage: whole
+    waves, whole 32-butterfly groups, the same number of groups per wave."""
+    if e.precision != 32 or 16 not in e.stages:
+        return False
+    nb = e.length // 16
+    waves, groups = e.threads // 64, (e.cols * nb) // 32
+    return (e.threads % 64 == 0 and (e.cols * nb) % 32 == 0 and waves > 0 and
+            groups % waves == 0)
+
+
 def _variants(e: Entry) -> list[Entry]:
     """Bank 1 (twice the columns) and bank 2 (half the columns) of a
-    power-of-two entry, where they fit the thread and LDS limits."""
-    if e.length < 64 or e.length & (e.length - 1):
-        return []
+    power-of-two entry, where they fit the thread and LDS limits; bank 3:
+    the same tile with its radix-16 sub-stages on the matrix cores (FP32
+    split-precision form), where the tile shape allows it."""
     out = []
+    m = Entry(e.precision, e.length, e.rows, e.tpt, e.cols, e.stages, MFMA_VARIANT)
+    if e.length & (e.length - 1) == 0 and mfma_fits(m):
+        out.append(m)
+    if e.length < 64 or e.length & (e.length - 1):
+        return out
     for variant, cols in ((1, e.cols * 2), (2, e.cols // 2)):
         if cols < 1 or (not e.rows and cols < 4):
             continue

@@ -280,6 +280,29 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
   }
 }
 
+/* Radix value that marks a radix-16 sub-stage evaluated on the matrix
+ * cores (MStage below) instead of by a VALU codelet. */
+constexpr int kMfma16 = -16;
+constexpr int radix_value(int r) { return r == kMfma16 ? 16 : r; }
+
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int R, int... REST>
+struct Stage;
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int... REST>
+struct MStage;
+
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int R, int... REST>
+struct StageSel {
+  using type = Stage<T, L, TPT, C, ROWS, NS, OFF, R, REST...>;
+};
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int... REST>
+struct StageSel<T, L, TPT, C, ROWS, NS, OFF, kMfma16, REST...> {
+  using type = MStage<T, L, TPT, C, ROWS, NS, OFF, REST...>;
+};
+
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
           int R, int... REST>
 struct Stage {
@@ -407,8 +430,9 @@ struct Stage {
       Dft<R>::template run<T>(vr[it], vi[it]);
     }
     if constexpr (!LAST) {
-      using Next = Stage<T, L, TPT, C, ROWS, NS * R,
-                         OFF + (NS > 1 ? NS * (R - 1) : 0), REST...>;
+      using Next = typename StageSel<T, L, TPT, C, ROWS, NS * R,
+                                     OFF + (NS > 1 ? NS * (R - 1) : 0),
+                                     REST...>::type;
       T nr[Next::IT][Next::RAD];
       T ni[Next::IT][Next::RAD];
 #if MOFFT_SPLIT_EXCHANGE
@@ -473,9 +497,9 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
 template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
 __device__ __forceinline__ void run_tile(const Ctx<T> &x,
                                          const mofft_pass_args &p) {
-  using First = Stage<T, L, TPT, C, ROWS, 1, 0, R0, R...>;
-  T vr[First::IT][R0];
-  T vi[First::IT][R0];
+  using First = typename StageSel<T, L, TPT, C, ROWS, 1, 0, R0, R...>::type;
+  T vr[First::IT][First::RAD];
+  T vi[First::IT][First::RAD];
   First::load_first(x, vr, vi);
   First::run(x, p, vr, vi);
 }
@@ -495,6 +519,7 @@ template <typename T, int L, int TPT, int C, bool ROWS, int... R>
 struct PassKernel {
   static constexpr int THREADS = TPT * C;
   static constexpr int STAGES = sizeof...(R);
+  static constexpr bool HAS_MFMA = ((R == kMfma16) || ...);
   static constexpr int LDS = Tile<T, L, TPT, C, ROWS>::BYTES;
   static int launch(const mofft_pass_args *a, hipStream_t s) {
     static bool configured = false;
@@ -513,11 +538,218 @@ struct PassKernel {
     if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
     /* LDS is only touched by sub-stage exchanges and by the transposed
      * store of strided passes whose output interleaves < C columns */
-    const bool needs_lds = STAGES > 1 || (!ROWS && a->ns < C);
+    const bool needs_lds = STAGES > 1 || HAS_MFMA || (!ROWS && a->ns < C);
     hipLaunchKernelGGL((fft_pass_kernel<T, L, TPT, C, ROWS, R...>),
                        dim3((unsigned)blocks), dim3(THREADS),
                        needs_lds ? LDS : 0, s, *a);
     return (int)hipGetLastError();
+  }
+};
+
+
+/* ------------------------------------------------------------------ */
+/* radix-16 sub-stage on the matrix cores (FP32 plans, kernel bank 3)   */
+/* ------------------------------------------------------------------ */
+
+typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
+typedef float mf_f16 __attribute__((ext_vector_type(16)));
+
+/* A radix-16 DFT of 32 butterflies is one real 32x32 by 32x32 product
+ * D = M V: column j of V holds butterfly j's inputs (rows 0-15 real parts,
+ * 16-31 imaginary parts), M is the real form of the 16-point DFT matrix
+ * (mofft_mfma16_a, generated), column j of D the 16 outputs.  It runs as
+ * v_mfma_f32_32x32x16_f16 products in split precision: every operand is
+ * a sum hi + lo of two FP16 values, D = Mh Vh + Mh Vl + Ml Vh accumulated
+ * in FP32 (the dropped Ml Vl is 2^-22 relative).  Each butterfly's inputs
+ * are first scaled by a power of two that puts their largest magnitude in
+ * [2^14, 2^15) (FP16 range for hi, normal range for lo); the outputs are
+ * scaled back exactly.
+ *
+ * Operand layouts (gfx950, verified on the device): A[i][k] in lane
+ * i%32 + 32*(k/8) element k%8; B[k][j] in lane j + 32*(k/8) element k%8;
+ * D[i][j] in lane j + 32*((i/4)%2) register (i/8)*4 + i%4.  So lane
+ * (j, h = lane/32) holds inputs m = 8h + e (e < 8) of butterfly j and,
+ * after the products, outputs q = 8*(e/4) + 4h + e%4 (real in register e,
+ * imaginary in register 8 + e).
+ *
+ * The stage reads its inputs from the LDS tile and leaves its outputs in
+ * the tile like any other sub-stage; as the first sub-stage it loads from
+ * HBM in the thread layout of a VALU radix-16 stage and exchanges through
+ * LDS, as the last it exchanges back to that layout and stores from it. */
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int... REST>
+struct MStage {
+  static_assert(sizeof(T) == 4, "matrix-core stages are FP32");
+  using TL = Tile<T, L, TPT, C, ROWS>;
+  static constexpr int R = 16;
+  static constexpr int RAD = 8;            /* values per lane and group */
+  static constexpr int NB = L / R;
+  static constexpr int NT = TPT * C;
+  static constexpr int WAVES = NT / 64;
+  static constexpr int GROUPS = (C * NB) / 32;
+  static constexpr int IT = GROUPS / (WAVES > 0 ? WAVES : 1);
+  static constexpr bool LAST = sizeof...(REST) == 0;
+  static_assert(NT % 64 == 0 && (C * NB) % 32 == 0 && GROUPS % WAVES == 0,
+                "matrix-core tile shape");
+  /* the VALU radix-16 stage whose thread layout HBM loads / stores use */
+  using V = Stage<T, L, TPT, C, ROWS, NS, OFF, 16>;
+
+  static __device__ __forceinline__ void where(const Ctx<T> &x, int it,
+                                               int &c, int &jj, int &h) {
+    const int lane = x.tid & 63;
+    const int g = ((x.tid >> 6) + it * WAVES) * 32 + (lane & 31);
+    c = g / NB;
+    jj = g - c * NB;
+    h = lane >> 5;
+  }
+
+  static __device__ __forceinline__ void load_lds(const Ctx<T> &x,
+                                                  const T *arr,
+                                                  T (&v)[IT][RAD]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int c, jj, h;
+      where(x, it, c, jj, h);
+#pragma unroll
+      for (int e = 0; e < RAD; ++e)
+        v[it][e] = arr[TL::stage_addr(c, jj + (8 * h + e) * NB)];
+    }
+  }
+
+  static __device__ __forceinline__ void put_lds(const Ctx<T> &x, T *arr,
+                                                 T (&v)[IT][RAD]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int c, jj, h;
+      where(x, it, c, jj, h);
+      const int base = (jj / NS) * NS * R + (jj % NS);
+#pragma unroll
+      for (int e = 0; e < RAD; ++e) {
+        const int q = 8 * (e >> 2) + 4 * h + (e & 3);
+        arr[TL::stage_addr(c, base + q * NS)] = v[it][e];
+      }
+    }
+  }
+
+  /* thread-layout values of V at their positions jj + k*NB */
+  static __device__ __forceinline__ void put_natural(const Ctx<T> &x, T *arr,
+                                                     T (&u)[V::IT][16]) {
+#pragma unroll
+    for (int it = 0; it < V::IT; ++it) {
+      const int jj = x.t + it * TPT;
+      if (V::FULL || jj < NB) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+          arr[TL::stage_addr(x.c, jj + k * NB)] = u[it][k];
+      }
+    }
+  }
+
+  static __device__ __forceinline__ void load_first(const Ctx<T> &x,
+                                                    T (&vr)[IT][RAD],
+                                                    T (&vi)[IT][RAD]) {
+    T ur[V::IT][16];
+    T ui[V::IT][16];
+    V::load_first(x, ur, ui);
+    put_natural(x, x.lds, ur);
+    __syncthreads();
+    load_lds(x, x.lds, vr);
+    __syncthreads();
+    put_natural(x, x.lds, ui);
+    __syncthreads();
+    load_lds(x, x.lds, vi);
+    __syncthreads();
+  }
+
+  static __device__ __forceinline__ void dft(const Ctx<T> &x, int it,
+                                             T (&re)[RAD], T (&im)[RAD]) {
+    int c, jj, h;
+    where(x, it, c, jj, h);
+    if constexpr (NS > 1) {
+      const cx<T> *w = x.stw + OFF + (jj % NS) * (R - 1);
+#pragma unroll
+      for (int e = 0; e < RAD; ++e) {
+        const int m = 8 * h + e;
+        if (m > 0) cmul<T>(re[e], im[e], w[m - 1]);
+      }
+    }
+    float mx = 0.f;
+#pragma unroll
+    for (int e = 0; e < RAD; ++e)
+      mx = fmaxf(mx, fmaxf(fabsf(re[e]), fabsf(im[e])));
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    int ex = 0;
+    (void)frexpf(mx, &ex);
+    int sh = 15 - ex;
+    sh = sh > 100 ? 100 : (sh < -100 ? -100 : sh);
+    const float up = ldexpf(1.f, sh);
+    const float down = ldexpf(1.f, -sh);
+    mf_h8 rh, rl, ih, il;
+#pragma unroll
+    for (int e = 0; e < RAD; ++e) {
+      const float a = re[e] * up;
+      const float b = im[e] * up;
+      const _Float16 ah = (_Float16)a;
+      const _Float16 bh = (_Float16)b;
+      rh[e] = ah;
+      ih[e] = bh;
+      rl[e] = (_Float16)(a - (float)ah);
+      il[e] = (_Float16)(b - (float)bh);
+    }
+    const int lane = x.tid & 63;
+    const mf_h8 *tab = reinterpret_cast<const mf_h8 *>(mofft_mfma16_a);
+    const mf_h8 mh0 = tab[0 * 64 + lane], mh1 = tab[1 * 64 + lane];
+    const mf_h8 ml0 = tab[2 * 64 + lane], ml1 = tab[3 * 64 + lane];
+    mf_f16 acc = {};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml0, rh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml1, ih, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh0, rl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh1, il, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh0, rh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh1, ih, acc, 0, 0, 0);
+#pragma unroll
+    for (int e = 0; e < RAD; ++e) {
+      re[e] = acc[e] * down;
+      im[e] = acc[8 + e] * down;
+    }
+  }
+
+  static __device__ __forceinline__ void run(const Ctx<T> &x,
+                                             const mofft_pass_args &p,
+                                             T (&vr)[IT][RAD],
+                                             T (&vi)[IT][RAD]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) dft(x, it, vr[it], vi[it]);
+    if constexpr (!LAST) {
+      using Next = typename StageSel<T, L, TPT, C, ROWS, NS * R,
+                                     OFF + (NS > 1 ? NS * (R - 1) : 0),
+                                     REST...>::type;
+      T nr[Next::IT][Next::RAD];
+      T ni[Next::IT][Next::RAD];
+      put_lds(x, x.lds, vr);
+      __syncthreads();
+      Next::load_lds(x, x.lds, nr);
+      __syncthreads();
+      put_lds(x, x.lds, vi);
+      __syncthreads();
+      Next::load_lds(x, x.lds, ni);
+      __syncthreads();
+      Next::run(x, p, nr, ni);
+    } else {
+      /* back to V's thread layout (outputs at jj + q*NB), then store */
+      T ur[V::IT][16];
+      T ui[V::IT][16];
+      put_lds(x, x.lds, vr);
+      __syncthreads();
+      V::load_lds(x, x.lds, ur);
+      __syncthreads();
+      put_lds(x, x.lds, vi);
+      __syncthreads();
+      V::load_lds(x, x.lds, ui);
+      __syncthreads();
+      if (ROWS || p.ns >= C) V::store_direct(x, ur, ui);
+      else store_tile<T, L, TPT, C, ROWS, V::IT, 16>(x, p, ur, ui);
+    }
   }
 };
 

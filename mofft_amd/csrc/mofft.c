@@ -47,6 +47,8 @@
  * MOFFT_CHUNK_BYTES in the environment overrides it (a tuning knob; read
  * once per process). */
 #define MOFFT_DEFAULT_CHUNK_BYTES ((size_t)256 << 20)
+/* Kernel bank of the matrix-core (split FP16) radix-16 kernels. */
+#define MOFFT_MFMA_BANK 3u
 
 static size_t chunk_bytes(void) {
   static size_t cached = 0;
@@ -757,7 +759,10 @@ static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
                           size_t length, size_t batch, int precision_bits,
                           const int *radices, size_t depth, uint64_t mask) {
   mofft_schedule base;
+  /* the matrix-core bank computes in split FP16 (other rounding): MEASURE
+   * only times numerically equivalent plain plans */
   for (unsigned int bank = 0; bank < kernel_bank_count(); ++bank) {
+    if (bank == MOFFT_MFMA_BANK) continue;
     if (*count >= cap) return;
     mofft_schedule s;
     if (build_schedule(PASS_KERNELS, length, batch, precision_bits, radices,
@@ -1068,10 +1073,50 @@ mofft_status mofft_plan_create_with_options_f64(
                              direction, placement, options, diagnostics);
 }
 
+/* MOFFT_PLAN_MFMA (FP32): the analytical chain (its passes are the
+ * radix-16-led registry kernels) on the matrix-core kernel bank. */
+static mofft_status create_mfma(mofft_plan_common **out, size_t length,
+                                size_t batch, mofft_direction direction,
+                                mofft_placement placement,
+                                mofft_plan_diagnostics *diagnostics) {
+  if (!out) return MOFFT_INVALID_ARGUMENT;
+  *out = NULL;
+  if (diagnostics) memset(diagnostics, 0, sizeof(*diagnostics));
+  if ((direction != MOFFT_FORWARD && direction != MOFFT_BACKWARD) ||
+      (placement != MOFFT_IN_PLACE && placement != MOFFT_OUT_OF_PLACE) ||
+      batch == 0)
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
+  int radices[MOFFT_MAX_PLAN_DEPTH];
+  size_t depth = 0;
+  uint64_t mask = 0;
+  const double start = monotonic_seconds();
+  mofft_status st = model_chain(PASS_KERNELS, length, 32, radices, &depth, &mask);
+  if (st != MOFFT_SUCCESS) return st;
+  st = plan_from_chain(out, length, batch, 32, direction, placement, radices,
+                       depth, mask, MOFFT_MFMA_BANK);
+  if (st != MOFFT_SUCCESS) return st;
+  if (diagnostics) {
+    diagnostics->used_model_fallback = 1;
+    fill_diagnostics(diagnostics, *out);
+    diagnostics->planning_seconds = monotonic_seconds() - start;
+  }
+  return MOFFT_SUCCESS;
+}
+
 mofft_status mofft_plan_create_batched_f32(
     mofft_plan_f32 **plan, size_t length, size_t batch,
     mofft_direction direction, mofft_placement placement,
     const mofft_plan_options *options, mofft_plan_diagnostics *diagnostics) {
+  if (options && (options->flags & MOFFT_PLAN_MFMA)) {
+    if (options->mode != MOFFT_PLANNER_PRETUNED ||
+        (options->flags & ~(uint32_t)MOFFT_PLAN_MFMA) != 0) {
+      if (plan) *plan = NULL;
+      return MOFFT_INVALID_ARGUMENT;
+    }
+    return create_mfma((mofft_plan_common **)(void *)plan, length, batch,
+                       direction, placement, diagnostics);
+  }
   return create_with_options((mofft_plan_common **)(void *)plan, length, batch,
                              32, direction, placement, options, diagnostics);
 }
