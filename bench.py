@@ -11,8 +11,12 @@ Prints ONE JSON line:
   roofline   dominant kernel vs the 8 TB/s HBM3E peak.  Algorithmic bytes are
              one read + one write of every point per transform direction
              (2*N*16 B per FP64 transform); a plan with P HBM passes is
-             charged 1/P of that per pass launch, so `frac` is the
-             end-to-end fraction of the HBM roofline.  `traffic` is the
+             charged 1/P of that per pass launch.  The dominant kernel is
+             the pass with the longest launches, timed with HIP events
+             between consecutive pass launches on the launch stream over
+             the same number of executions as the timed region;
+             `achieved` = algorithmic bytes per launch / its mean launch
+             time.  `traffic` is the
              measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE,
              profiles/), or null when no counter summary is present.
   cpu_baseline  the reference algorithm restated in C (oracle/), one core,
@@ -176,8 +180,6 @@ def run_gpu(args):
     passes = desc.pass_count
     chunks = math.ceil(batch / desc.chunk_transforms)
     launches_per_step = sum(p.description().pass_count * chunks for p, _, _ in plans)
-    kinfo = mf.kernels()[desc.pass_kernel[0]]
-    kernel_key = f"f{bits}_L{kinfo.length}_{'rows' if kinfo.rows else 'cols'}"
 
     def step():
         for p, a, b in plans:
@@ -207,9 +209,22 @@ def run_gpu(args):
     gflops = flops / (ms * 1e-3) / 1e9
     cb = 8 if bits == 32 else 16
     alg_bytes_step = directions * 2.0 * n * batch * cb
-    launch_ms = total_ms / (args.steps * launches_per_step)
     alg_per_launch = alg_bytes_step / launches_per_step
-    achieved = alg_per_launch / (launch_ms * 1e-3) / 1e9
+    # per-kernel launch durations: the same executions with a device
+    # timestamp between consecutive pass launches (HIP events on the launch
+    # stream); the dominant kernel is the pass with the longest launches
+    per_pass = [0.0] * passes
+    for _ in range(args.steps):
+        for p, a, b in plans:
+            for i, sec in enumerate(p.execute_timed(a, b, sh)):
+                per_pass[i] += sec
+    launches_per_pass = args.steps * len(plans) * chunks
+    pass_us = [1e6 * v / launches_per_pass for v in per_pass]
+    dom = max(range(passes), key=lambda i: pass_us[i])
+    kinfo = mf.kernels()[desc.pass_kernel[dom]]
+    kernel_key = f"f{bits}_L{kinfo.length}_{'rows' if kinfo.rows else 'cols'}"
+    launch_us = pass_us[dom]
+    achieved = alg_per_launch / (launch_us * 1e-6) / 1e9
     traffic = load_traffic(kernel_key)
     copy_gbs = measure_copy(torch, dev, stream, 256 << 20)
     line = {
@@ -237,10 +252,32 @@ def run_gpu(args):
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "kernel": kernel_key,
+                     "kernel_template": f"fft_pass_kernel<{'float' if bits == 32 else 'double'}, "
+                                        f"{kinfo.length}, {kinfo.threads_per_column}, "
+                                        f"{kinfo.columns_per_block}, {'true' if kinfo.rows else 'false'}, "
+                                        + ", ".join(str(r) for r in kinfo.stages[:kinfo.stage_count]) + ">",
                      "launches_per_step": launches_per_step,
-                     "avg_launch_us": round(launch_ms * 1e3, 3)},
+                     "avg_launch_us": round(launch_us, 3),
+                     "pass_launch_us": [round(v, 3) for v in pass_us]},
         "wall_s": round(wall, 3),
     }
+    if args.workload == "c1":
+        # the same execution replayed from a captured HIP graph
+        p0, a0, b0 = plans[0]
+        with p0.capture(a0, b0) as gr:
+            for _ in range(args.warmup):
+                gr.launch(sh)
+            torch.cuda.synchronize()
+            gs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(args.steps)]
+            for e0, e1 in gs:
+                e0.record(stream)
+                gr.launch(sh)
+                e1.record(stream)
+            torch.cuda.synchronize()
+            g_ms = [e0.elapsed_time(e1) for e0, e1 in gs]
+        line["graph_p50_ms"] = round(percentile(g_ms, 0.5), 4)
+        line["graph_p99_ms"] = round(percentile(g_ms, 0.99), 4)
     for p, _, _ in plans:
         p.destroy()
     return line, (n, bits)

@@ -82,6 +82,35 @@ mofft_status mofft_execute_async_workspace_f64(
     mofft_complex_f64 *output, void *workspace, size_t workspace_bytes,
     void *stream);
 
+/* A plan's launches on fixed device buffers captured into a HIP graph: one
+ * replay per execution, for launch-bound loops.  The graph uses the plan's
+ * twiddles and workspace (destroy it before the plan); replays of one graph
+ * must be ordered on one stream. */
+typedef struct mofft_graph mofft_graph;
+mofft_status mofft_plan_graph_create_f32(const mofft_plan_f32 *plan,
+                                         const mofft_complex_f32 *input,
+                                         mofft_complex_f32 *output,
+                                         mofft_graph **graph);
+mofft_status mofft_plan_graph_create_f64(const mofft_plan_f64 *plan,
+                                         const mofft_complex_f64 *input,
+                                         mofft_complex_f64 *output,
+                                         mofft_graph **graph);
+mofft_status mofft_graph_launch(const mofft_graph *graph, void *stream);
+void mofft_graph_destroy(mofft_graph *graph);
+
+/* Execute once (device buffers) with a device timestamp between
+ * consecutive pass launches, wait, and return in pass_seconds[p] the time of
+ * pass p summed over the batch chunks (capacity >= the plan's pass count).
+ * For profiling; the kernels are those of mofft_execute_async_*. */
+mofft_status mofft_execute_timed_f32(const mofft_plan_f32 *plan,
+                                     const mofft_complex_f32 *input,
+                                     mofft_complex_f32 *output, void *stream,
+                                     double *pass_seconds, size_t capacity);
+mofft_status mofft_execute_timed_f64(const mofft_plan_f64 *plan,
+                                     const mofft_complex_f64 *input,
+                                     mofft_complex_f64 *output, void *stream,
+                                     double *pass_seconds, size_t capacity);
+
 /* The GPU schedule of a plan: the chain grouped into HBM passes.  Pass 0 is
  * executed first.  Each pass is one kernel launch per batch chunk that keeps
  * `pass_length` points of every column in LDS. */

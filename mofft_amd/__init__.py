@@ -209,6 +209,9 @@ for _s in ("f32", "f64"):
         (f"mofft_plan_workspace_size_{_s}", _I, [_P, ctypes.POINTER(_SZ)]),
         (f"mofft_execute_async_workspace_{_s}", _I, [_P, _P, _P, _P, _SZ, _P]),
         (f"mofft_plan_get_description_{_s}", _I, [_P, ctypes.POINTER(PlanDescription)]),
+        (f"mofft_plan_graph_create_{_s}", _I, [_P, _P, _P, ctypes.POINTER(_P)]),
+        (f"mofft_execute_timed_{_s}", _I,
+         [_P, _P, _P, _P, ctypes.POINTER(ctypes.c_double), _SZ]),
     ]
 _SIGS += [
     ("mofft_plan_options_init", None, [ctypes.POINTER(PlanOptions)]),
@@ -230,6 +233,8 @@ _SIGS += [
     ("mofft_kernel_count", _I, []),
     ("mofft_kernel_get_info", _I, [_I, ctypes.POINTER(KernelInfo)]),
     ("mofft_device_available", _I, []),
+    ("mofft_graph_launch", _I, [_P, _P]),
+    ("mofft_graph_destroy", None, [_P]),
     ("mofft_batch_options_init", None, [ctypes.POINTER(BatchOptions)]),
     ("mofft_batch_plan_create", _I,
      [ctypes.POINTER(_P), _I, _SZ, ctypes.POINTER(_SZ), ctypes.POINTER(_I),
@@ -408,6 +413,22 @@ class Plan:
             _ptr(workspace) if workspace is not None else None, nbytes,
             stream or None), "execute_async_workspace")
 
+    def capture(self, x_in, x_out) -> "Graph":
+        """The plan's launches on these device buffers as a HIP graph."""
+        g = ctypes.c_void_p()
+        _check(self._fn("mofft_plan_graph_create")(self._handle, _ptr(x_in), _ptr(x_out),
+                                                   ctypes.byref(g)), "plan_graph_create")
+        return Graph(g)
+
+    def execute_timed(self, x_in, x_out, stream: int = 0) -> list[float]:
+        """One execution with device timestamps between pass launches;
+        returns seconds per pass (summed over batch chunks)."""
+        secs = (ctypes.c_double * MAX_PASSES)()
+        _check(self._fn("mofft_execute_timed")(self._handle, _ptr(x_in), _ptr(x_out),
+                                               stream or None, secs, MAX_PASSES),
+               "execute_timed")
+        return list(secs[: self.description().pass_count])
+
     def export(self) -> str:
         need = ctypes.c_size_t()
         _check(self._fn("mofft_plan_export")(self._handle, None, 0, ctypes.byref(need)),
@@ -421,6 +442,33 @@ class Plan:
         if self._handle and self._handle.value:
             self._fn("mofft_plan_destroy")(self._handle)
             self._handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class Graph:
+    """A captured plan execution (mofft_plan_graph_create_*)."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    def launch(self, stream: int = 0) -> None:
+        _check(lib().mofft_graph_launch(self._h, stream or None), "graph_launch")
+
+    def destroy(self) -> None:
+        if self._h and self._h.value:
+            lib().mofft_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:

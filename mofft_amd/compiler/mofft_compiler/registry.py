@@ -145,6 +145,7 @@ def _make(precision: int, length: int, rows: int,
 
 
 MFMA_VARIANT = 3
+WIDE_VARIANT = 4
 
 
 def mfma_fits(e: Entry) -> bool:
@@ -164,6 +165,11 @@ def _variants(e: Entry) -> list[Entry]:
     the same tile with its radix-16 sub-stages on the matrix cores (FP32
     split-precision form), where the tile shape allows it."""
     out = []
+    if not e.rows and 64 <= e.length <= 256 and e.length & (e.length - 1) == 0:
+        # bank 4: four times the columns (512-byte+ runs per access)
+        w = Entry(e.precision, e.length, e.rows, e.tpt, e.cols * 4, e.stages, WIDE_VARIANT)
+        if w.threads <= 1024 and w.lds_bytes <= LDS_CAP:
+            out.append(w)
     m = Entry(e.precision, e.length, e.rows, e.tpt, e.cols, e.stages, MFMA_VARIANT)
     if e.length & (e.length - 1) == 0 and mfma_fits(m):
         out.append(m)

@@ -23,6 +23,9 @@ from pathlib import Path
 RADICES = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 32, 64}
 
 
+PLAIN_BANKS = (0, 1, 2, 4)
+
+
 def load_entries(path: str | Path) -> tuple[list[dict], bool, str]:
     payload = json.loads(Path(path).read_text())
     if payload.get("schema_version") != 1:
@@ -61,8 +64,10 @@ def load_entries(path: str | Path) -> tuple[list[dict], bool, str]:
         if layout not in ("transpose", "section"):
             raise ValueError(f"invalid stage layout {layout!r}")
         bank = entry.get("kernel_bank", 0)
-        if isinstance(bank, bool) or not isinstance(bank, int) or not 0 <= bank <= 3:
-            raise ValueError("kernel_bank must be an integer from 0 to 3")
+        # tile-shape banks of the registry; bank 3 (matrix-core split
+        # precision) rounds differently and is never a PRETUNED choice
+        if isinstance(bank, bool) or not isinstance(bank, int) or bank not in PLAIN_BANKS:
+            raise ValueError(f"kernel_bank must be one of {PLAIN_BANKS}")
         out.append({"length": length, "precision": precision, "radices": radices,
                     "mask": mask, "direct": int(direct),
                     "layout": int(layout == "section"), "bank": bank})

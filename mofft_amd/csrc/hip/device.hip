@@ -274,6 +274,36 @@ int mofft_hip_batch_graph(int precision_bits, const mofft_batch_header *h,
   return (int)e;
 }
 
+int mofft_hip_capture_begin(void **stream) {
+  if (!stream) return (int)hipErrorInvalidValue;
+  *stream = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return (int)e;
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(s);
+    return (int)e;
+  }
+  *stream = (void *)s;
+  return 0;
+}
+
+int mofft_hip_capture_end(void *stream, void **graph_exec) {
+  if (!stream || !graph_exec) return (int)hipErrorInvalidValue;
+  *graph_exec = nullptr;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g);
+  if (e == hipSuccess) {
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e == hipSuccess) *graph_exec = (void *)x;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy((hipStream_t)stream);
+  return (int)e;
+}
+
 int mofft_hip_graph_launch(void *graph_exec, void *stream) {
   if (!graph_exec) return (int)hipErrorInvalidValue;
   return (int)hipGraphLaunch((hipGraphExec_t)graph_exec, (hipStream_t)stream);
@@ -302,6 +332,22 @@ int mofft_hip_event_create(void **event) {
   hipEvent_t ev = nullptr;
   const hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
   *event = e == hipSuccess ? (void *)ev : nullptr;
+  return (int)e;
+}
+
+int mofft_hip_timing_event_create(void **event) {
+  if (!event) return (int)hipErrorInvalidValue;
+  hipEvent_t ev = nullptr;
+  const hipError_t e = hipEventCreate(&ev);
+  *event = e == hipSuccess ? (void *)ev : nullptr;
+  return (int)e;
+}
+
+int mofft_hip_event_elapsed(void *start, void *end, double *seconds) {
+  if (!seconds) return (int)hipErrorInvalidValue;
+  float ms = 0.f;
+  const hipError_t e = hipEventElapsedTime(&ms, (hipEvent_t)start, (hipEvent_t)end);
+  *seconds = (double)ms * 1e-3;
   return (int)e;
 }
 

@@ -47,8 +47,10 @@
  * MOFFT_CHUNK_BYTES in the environment overrides it (a tuning knob; read
  * once per process). */
 #define MOFFT_DEFAULT_CHUNK_BYTES ((size_t)256 << 20)
-/* Kernel bank of the matrix-core (split FP16) radix-16 kernels. */
+/* Kernel bank of the matrix-core (split FP16) radix-16 kernels, and of
+ * four-times-wider strided tiles. */
 #define MOFFT_MFMA_BANK 3u
+#define MOFFT_WIDE_BANK 4u
 
 static size_t chunk_bytes(void) {
   static size_t cached = 0;
@@ -63,6 +65,24 @@ static size_t chunk_bytes(void) {
     cached = v;
   }
   return cached;
+}
+
+/* Largest full pass-twiddle table (bytes); MOFFT_PASS_TABLE_BYTES in the
+ * environment overrides it (a tuning knob; 0 = always the two-level
+ * table; read once per process). */
+static size_t pass_table_bytes(void) {
+  static int cached = 0;
+  static size_t v = MOFFT_PASS_TABLE_BYTES;
+  if (!cached) {
+    const char *env = getenv("MOFFT_PASS_TABLE_BYTES");
+    if (env && *env) {
+      char *end = NULL;
+      const unsigned long long x = strtoull(env, &end, 10);
+      if (end && *end == '\0') v = (size_t)x;
+    }
+    cached = 1;
+  }
+  return v;
 }
 
 /* src/mofft.c:27-28 - the radix set the API accepts. */
@@ -262,7 +282,14 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
     for (size_t len = limit; len >= 1; --len) {
       const int rows = (start == 0 && len == depth) ? 1 : 0;
       if (len > 8) continue;
-      kernel = find_kernel(ks, precision_bits, rows, exec + start, (int)len, bank);
+      /* the wide bank widens only the passes after the first: their
+       * stores are column runs, the first pass leaves through LDS as
+       * whole blocks and prefers the occupancy of the narrow tile
+       * (measured, FP64 N = 2^16: 130 vs 171 us for pass 0, 139 vs
+       * 161 us for pass 1) */
+      const unsigned int b =
+          (bank == MOFFT_WIDE_BANK && s->pass_count == 0) ? 0u : bank;
+      kernel = find_kernel(ks, precision_bits, rows, exec + start, (int)len, b);
       if (kernel >= 0) { take = len; break; }
     }
     if (kernel < 0) return MOFFT_UNSUPPORTED_SIZE;
@@ -296,7 +323,7 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   for (int i = 1; i < s->pass_count; ++i) {
     mofft_pass *p = &s->passes[i];
     const size_t entries = p->span * p->length;
-    if (sat_mul(entries, cb0) <= MOFFT_PASS_TABLE_BYTES) {
+    if (sat_mul(entries, cb0) <= pass_table_bytes()) {
       p->pass_table = 1;
       p->pass_offset = count;
       count += entries;
@@ -588,9 +615,27 @@ static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
 /* execution                                                            */
 /* ------------------------------------------------------------------ */
 
+/* Optional per-launch timing: events[k] is recorded before launch k and
+ * events[k + 1] after it (k counts launches over chunks and passes). */
+typedef struct {
+  void **events;
+  size_t capacity;
+  size_t used;
+} launch_timing;
+
+static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
+                                 void *out, void *work, void *stream,
+                                 size_t count, launch_timing *timing);
+
 static int launch_schedule_count(const mofft_plan_common *p, const void *in,
                                  void *out, void *work, void *stream,
                                  size_t count) {
+  return launch_schedule_timed(p, in, out, work, stream, count, NULL);
+}
+
+static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
+                                 void *out, void *work, void *stream,
+                                 size_t count, launch_timing *timing) {
   const mofft_schedule *s = &p->s;
   const size_t cb = complex_bytes(s->precision_bits);
   const char *tw = (const char *)p->twiddles;
@@ -631,8 +676,17 @@ static int launch_schedule_count(const mofft_plan_common *p, const void *in,
       if (backward && i == P - 1) a.flags |= MOFFT_PASS_SWAP_OUT;
       if (ps->span > 1) a.flags |= MOFFT_PASS_TWIDDLE;
       if (ps->span > 1 && ps->pass_table) a.flags |= MOFFT_PASS_TWIDDLE_TABLE;
+      if (timing && timing->used == 0) {
+        if (timing->capacity < 1) return 1;
+        if (mofft_hip_event_record(timing->events[0], stream) != 0) return 1;
+      }
       const int rc = mofft_hip_launch_pass(ps->kernel, &a, stream);
       if (rc != 0) return rc;
+      if (timing) {
+        if (timing->used + 2 > timing->capacity) return 1;
+        if (mofft_hip_event_record(timing->events[++timing->used], stream) != 0)
+          return 1;
+      }
       src = dst;
     }
   }
@@ -759,6 +813,7 @@ static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
                           size_t length, size_t batch, int precision_bits,
                           const int *radices, size_t depth, uint64_t mask) {
   mofft_schedule base;
+  memset(&base, 0, sizeof(base));
   /* the matrix-core bank computes in split FP16 (other rounding): MEASURE
    * only times numerically equivalent plain plans */
   for (unsigned int bank = 0; bank < kernel_bank_count(); ++bank) {
@@ -1988,4 +2043,118 @@ mofft_status mofft_internal_execute_count(const void *plan, const void *in,
   return launch_schedule_count(p, in, out, work, stream, count) == 0
              ? MOFFT_SUCCESS
              : MOFFT_INTERNAL_ERROR;
+}
+
+/* Execute once with a timestamp between consecutive pass launches; the
+ * per-launch durations are summed per pass index into pass_seconds. */
+static mofft_status execute_timed(const mofft_plan_common *p, const void *in,
+                                  void *out, void *stream,
+                                  double *pass_seconds, size_t capacity) {
+  if (!p || !in || !out || !pass_seconds ||
+      capacity < (size_t)p->s.pass_count)
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_is_device_pointer(in) || !mofft_hip_is_device_pointer(out))
+    return MOFFT_INVALID_ARGUMENT;
+  if ((p->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  const size_t chunks = (p->s.batch + p->s.chunk - 1) / p->s.chunk;
+  const size_t n_events = 1 + chunks * (size_t)p->s.pass_count;
+  void **events = (void **)calloc(n_events, sizeof(void *));
+  if (!events) return MOFFT_ALLOCATION_FAILURE;
+  mofft_status st = MOFFT_SUCCESS;
+  for (size_t k = 0; k < n_events && st == MOFFT_SUCCESS; ++k)
+    if (mofft_hip_timing_event_create(&events[k]) != 0) st = MOFFT_INTERNAL_ERROR;
+  launch_timing t = {events, n_events, 0};
+  if (st == MOFFT_SUCCESS &&
+      launch_schedule_timed(p, in, out, p->workspace, stream, p->s.batch, &t) != 0)
+    st = MOFFT_INTERNAL_ERROR;
+  if (st == MOFFT_SUCCESS && mofft_hip_synchronize(stream) != 0)
+    st = MOFFT_INTERNAL_ERROR;
+  for (int i = 0; i < p->s.pass_count; ++i) pass_seconds[i] = 0.0;
+  for (size_t k = 0; k + 1 <= t.used && st == MOFFT_SUCCESS; ++k) {
+    double sec = 0.0;
+    if (mofft_hip_event_elapsed(events[k], events[k + 1], &sec) != 0)
+      st = MOFFT_INTERNAL_ERROR;
+    pass_seconds[k % (size_t)p->s.pass_count] += sec;
+  }
+  for (size_t k = 0; k < n_events; ++k) mofft_hip_event_destroy(events[k]);
+  free(events);
+  return st;
+}
+
+mofft_status mofft_execute_timed_f32(const mofft_plan_f32 *plan,
+                                     const mofft_complex_f32 *input,
+                                     mofft_complex_f32 *output, void *stream,
+                                     double *pass_seconds, size_t capacity) {
+  return execute_timed(plan ? &plan->c : NULL, input, output, stream,
+                       pass_seconds, capacity);
+}
+
+mofft_status mofft_execute_timed_f64(const mofft_plan_f64 *plan,
+                                     const mofft_complex_f64 *input,
+                                     mofft_complex_f64 *output, void *stream,
+                                     double *pass_seconds, size_t capacity) {
+  return execute_timed(plan ? &plan->c : NULL, input, output, stream,
+                       pass_seconds, capacity);
+}
+
+/* ------------------------------------------------------------------ */
+/* captured plans (HIP graphs)                                          */
+/* ------------------------------------------------------------------ */
+
+struct mofft_graph {
+  void *exec;
+};
+
+static mofft_status graph_create(const mofft_plan_common *p, const void *in,
+                                 void *out, mofft_graph **graph) {
+  if (!graph) return MOFFT_INVALID_ARGUMENT;
+  *graph = NULL;
+  if (!p || !in || !out) return MOFFT_INVALID_ARGUMENT;
+  if ((p->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_is_device_pointer(in) || !mofft_hip_is_device_pointer(out))
+    return MOFFT_INVALID_ARGUMENT;
+  mofft_graph *g = (mofft_graph *)calloc(1, sizeof(*g));
+  if (!g) return MOFFT_ALLOCATION_FAILURE;
+  void *cs = NULL;
+  if (mofft_hip_capture_begin(&cs) != 0) {
+    free(g);
+    return MOFFT_INTERNAL_ERROR;
+  }
+  const int rc = launch_schedule(p, in, out, p->workspace, cs);
+  const int ec = mofft_hip_capture_end(cs, &g->exec);
+  if (rc != 0 || ec != 0 || !g->exec) {
+    mofft_hip_graph_destroy(g->exec);
+    free(g);
+    return MOFFT_INTERNAL_ERROR;
+  }
+  *graph = g;
+  return MOFFT_SUCCESS;
+}
+
+mofft_status mofft_plan_graph_create_f32(const mofft_plan_f32 *plan,
+                                         const mofft_complex_f32 *input,
+                                         mofft_complex_f32 *output,
+                                         mofft_graph **graph) {
+  return graph_create(plan ? &plan->c : NULL, input, output, graph);
+}
+
+mofft_status mofft_plan_graph_create_f64(const mofft_plan_f64 *plan,
+                                         const mofft_complex_f64 *input,
+                                         mofft_complex_f64 *output,
+                                         mofft_graph **graph) {
+  return graph_create(plan ? &plan->c : NULL, input, output, graph);
+}
+
+mofft_status mofft_graph_launch(const mofft_graph *graph, void *stream) {
+  if (!graph) return MOFFT_INVALID_ARGUMENT;
+  return mofft_hip_graph_launch(graph->exec, stream) == 0 ? MOFFT_SUCCESS
+                                                          : MOFFT_INTERNAL_ERROR;
+}
+
+void mofft_graph_destroy(mofft_graph *graph) {
+  if (!graph) return;
+  mofft_hip_graph_destroy(graph->exec);
+  free(graph);
 }

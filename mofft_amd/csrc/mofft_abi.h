@@ -159,6 +159,9 @@ int mofft_hip_batch_launch(int precision_bits, const mofft_batch_header *h,
 int mofft_hip_batch_graph(int precision_bits, const mofft_batch_header *h,
                           int grid, int lds_bytes, void **graph_exec);
 int mofft_hip_graph_launch(void *graph_exec, void *stream);
+/* Begin capturing a private stream; end the capture and instantiate. */
+int mofft_hip_capture_begin(void **stream);
+int mofft_hip_capture_end(void *stream, void **graph_exec);
 int mofft_hip_graph_destroy(void *graph_exec);
 
 /* ---- streams, events and device-side inputs (serving loop) ---- */
@@ -171,6 +174,10 @@ int mofft_hip_event_destroy(void *event);
 int mofft_hip_event_record(void *event, void *stream);
 /* 1 when the event has completed, 0 when not yet, negative on error. */
 int mofft_hip_event_query(void *event);
+/* An event that records timestamps, and the seconds between two of them
+ * (both complete). */
+int mofft_hip_timing_event_create(void **event);
+int mofft_hip_event_elapsed(void *start, void *end, double *seconds);
 int mofft_hip_event_synchronize(void *event);
 /* Interleaved complex values from a seed: component c (0 = real,
  * 1 = imaginary) of element i is 2u - 1 with u the top 53 bits of

@@ -95,12 +95,14 @@ def test_wisdom_compiler_validates(tmp_path):
             "blocked_transpose_mask": 0}
     text = wisdom.emit_c(_wisdom(tmp_path, [good]))
     assert "1024u, 32" in text
+    assert "4u, 3" in wisdom.emit_c(_wisdom(tmp_path, [{**good, "kernel_bank": 4}]))
     bad_cases = [
         {**good, "radices": [17, 16]},
         {**good, "radices": [4, 16, 8]},
         {**good, "blocked_transpose_mask": 4},
         {**good, "precision": "fp16"},
         {**good, "kernel_bank": 9},
+        {**good, "kernel_bank": 3},      # matrix-core split precision: never PRETUNED
     ]
     for bad in bad_cases:
         with pytest.raises((ValueError, KeyError)):

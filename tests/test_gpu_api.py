@@ -195,3 +195,33 @@ def test_twiddle_tables_rounded_once(gpu, torch_dev, bits):
     want = oracle.omega_table(n, -1, 64, indices=k)
     err = np.max(np.abs(y[0, k] - want))
     assert err <= (8 * np.finfo(np.float32).eps if bits == 32 else 8 * np.finfo(np.float64).eps)
+
+
+@pytest.mark.parametrize("bits,n,batch,in_place", [(32, 1024, 64, False), (64, 1 << 16, 4, True),
+                                                   (32, 1 << 21, 2, False)])
+def test_captured_graph_replays(gpu, torch_dev, bits, n, batch, in_place):
+    """A plan captured into a HIP graph: every replay (with eager work and
+    synchronisation between replays) equals the direct execution."""
+    torch = torch_dev
+    x = seeded_complex((batch, n), seed=n, dtype=np.complex64 if bits == 32 else np.complex128)
+    src = to_dev(torch, x, bits)
+    buf = src.clone() if in_place else src
+    out = buf if in_place else torch.empty_like(src)
+    with gpu.Plan(n, batch, bits, gpu.FORWARD, in_place=in_place) as p:
+        p.execute_async(buf, out, 0)
+        torch.cuda.synchronize()
+        want = out.clone()
+        with p.capture(buf, out) as g:
+            for rep in range(3):
+                if in_place:
+                    buf.copy_(src)
+                else:
+                    out.zero_()
+                junk = torch.randn(1 << 18, device="cuda").sum()
+                torch.cuda.synchronize()
+                g.launch(0)
+                torch.cuda.synchronize()
+                assert torch.equal(out, want), rep
+                del junk
+    ref = oracle.fft_dd(x, -1)
+    assert oracle.rel_l2(want.cpu().numpy(), ref) <= tol(n, bits)
