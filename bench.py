@@ -222,7 +222,8 @@ def run_gpu(args):
     pass_us = [1e6 * v / launches_per_pass for v in per_pass]
     dom = max(range(passes), key=lambda i: pass_us[i])
     kinfo = mf.kernels()[desc.pass_kernel[dom]]
-    kernel_key = f"f{bits}_L{kinfo.length}_{'rows' if kinfo.rows else 'cols'}"
+    kernel_key = (f"f{bits}_L{kinfo.length}_c{kinfo.columns_per_block}_"
+                  f"{'rows' if kinfo.rows else 'cols'}")
     launch_us = pass_us[dom]
     achieved = alg_per_launch / (launch_us * 1e-6) / 1e9
     traffic = load_traffic(kernel_key)
