@@ -131,8 +131,13 @@ def _make(precision: int, length: int, rows: int,
         # 128-thread workgroups measured best for whole-transform kernels
         cols = max(1, 128 // tpt)
     else:
-        # at least 128 B of adjacent columns per access, at least 256 threads
-        cols = max(16 if precision == 32 else 8, 256 // tpt)
+        # at least 128 B of adjacent columns per access, at least 256 threads;
+        # long strided columns give each thread several butterflies rather
+        # than narrowing the runs
+        cmin = 16 if precision == 32 else 8
+        while (tpt * cmin > 1024 and tpt > 1 and length & (length - 1) == 0):
+            tpt //= 2
+        cols = max(cmin, 256 // tpt)
         while cols * tpt > 1024 and cols > 1:
             cols //= 2
     e = Entry(precision, length, rows, tpt, cols, stages)
@@ -194,7 +199,7 @@ def entries() -> list[Entry]:
             out.append(e)
             out.extend(_variants(e))
 
-    for precision, rows_max, cols_max in ((32, 32768, 1024), (64, 16384, 1024)):
+    for precision, rows_max, cols_max in ((32, 32768, 4096), (64, 16384, 4096)):
         # single-pass kernels over whole transforms
         for k in range(1, rows_max.bit_length()):
             length = 1 << k
@@ -265,6 +270,9 @@ def check(es: list[Entry]) -> None:
             prod *= r
         assert prod == e.length, e
         assert e.threads <= 1024 and e.lds_bytes <= LDS_CAP, e
+        # the cooperative tile store gives every thread the same number of
+        # tile elements
+        assert e.rows or (e.cols * e.length) % e.threads == 0, e
     # every reference radix has a strided single-radix (or L = radix) entry
     for precision in (32, 64):
         for r in RADICES:

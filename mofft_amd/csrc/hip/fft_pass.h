@@ -205,6 +205,7 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
   constexpr int NB = L / R;
   constexpr bool FULL = IT * TPT == NB;
   constexpr int PER = (C * L) / TL::NT;
+  static_assert((C * L) % TL::NT == 0, "every thread stores PER elements");
   const int tid = x.tid;
   const int64_t ns = p.ns;
   const int span = (int)ns * L;

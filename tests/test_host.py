@@ -104,7 +104,7 @@ def test_no_device_means_unsupported_hardware():
 
 
 @pytest.mark.parametrize("n,bits,passes", [
-    (1 << 24, 32, [256, 256, 256]), (1 << 24, 64, [256, 256, 256]),
+    (1 << 25, 32, [512, 256, 256]), (1 << 25, 64, [512, 256, 256]),
     (45, 32, [45]), (20736, 64, [144, 144]), (371293, 32, [2197, 169]),
     (1728, 64, [1728]), (759375, 32, [3375, 225]),
 ])
