@@ -34,6 +34,14 @@ STAGE_RADICES = (16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2)
 # four-step factors
 MIXED_ROWS = (30, 45, 121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
+# strided kernels with radix-8 sub-stages besides the radix-16 chains: half
+# the registers per butterfly (FP64 radix-16 butterflies hold 64 VGPRs of
+# data), one more LDS exchange; the planner times both
+ALT_COLS = ((256, (8, 8, 4)), (1024, (8, 8, 4, 4)), (512, (8, 8, 8)),
+            (128, (8, 4, 4)), (2048, (8, 8, 8, 4)), (4096, (8, 8, 8, 8)))
+# the same for whole-transform kernels
+ALT_ROWS = ((1024, (8, 8, 4, 4)), (2048, (8, 8, 8, 4)), (4096, (8, 8, 8, 8)),
+            (8192, (8, 8, 8, 4, 4)), (16384, (8, 8, 8, 8, 4)))
 
 
 @dataclass(frozen=True)
@@ -208,6 +216,9 @@ def entries() -> list[Entry]:
             add(_make(precision, r, 1, (r,)))
         for length in MIXED_ROWS:
             add(_make(precision, length, 1, chain_for(length)))
+        for length, chain in ALT_ROWS:
+            if length <= rows_max:
+                add(_make(precision, length, 1, chain))
         # strided kernels for multi-pass schedules
         for k in range(1, cols_max.bit_length()):
             length = 1 << k
@@ -216,6 +227,8 @@ def entries() -> list[Entry]:
             add(_make(precision, r, 0, (r,)))
         for length in MIXED_COLS:
             add(_make(precision, length, 0, chain_for(length)))
+        for length, chain in ALT_COLS:
+            add(_make(precision, length, 0, chain))
     return out
 
 
