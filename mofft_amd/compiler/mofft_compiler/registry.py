@@ -34,14 +34,15 @@ STAGE_RADICES = (16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2)
 # four-step factors
 MIXED_ROWS = (30, 45, 121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
-# strided kernels with radix-8 sub-stages besides the radix-16 chains: half
-# the registers per butterfly (FP64 radix-16 butterflies hold 64 VGPRs of
-# data), one more LDS exchange; the planner times both
-ALT_COLS = ((256, (8, 8, 4)), (1024, (8, 8, 4, 4)), (512, (8, 8, 8)),
-            (128, (8, 4, 4)), (2048, (8, 8, 8, 4)), (4096, (8, 8, 8, 8)))
-# the same for whole-transform kernels
-ALT_ROWS = ((1024, (8, 8, 4, 4)), (2048, (8, 8, 8, 4)), (4096, (8, 8, 8, 8)),
-            (8192, (8, 8, 8, 4, 4)), (16384, (8, 8, 8, 8, 4)))
+# Besides the radix-16 chains, every power-of-two length from 64 has kernels
+# with radix-8 and with radix-4 sub-stages: half (a quarter)
+# of the registers per butterfly - an FP64 radix-16 butterfly holds 64 VGPRs
+# of data - for one more LDS exchange.  The planner times both; measured on
+# MI355X the radix-8 chains win for most multi-pass and for FP32 N=2^10..2^12
+# single-pass plans.
+ALT_RADIX4 = ((64, (4, 4, 4)), (256, (4, 4, 4, 4)), (512, (8, 4, 4, 4)),
+              (1024, (4, 4, 4, 4, 4)), (2048, (8, 4, 4, 4, 4)),
+              (4096, (4, 4, 4, 4, 4, 4)))
 
 
 @dataclass(frozen=True)
@@ -122,6 +123,20 @@ def _pow2_chain(length: int) -> tuple[int, ...]:
         n //= 16
     if n > 1:
         chain.append(n)
+    return tuple(chain)
+
+
+def _pow2_chain8(length: int) -> tuple[int, ...]:
+    """Radix-8 chain of a power of two >= 64: 8s, then the remainder, with a
+    trailing (8, 2) rebalanced to (4, 4)."""
+    chain = []
+    n = length
+    while n % 8 == 0 and n > 8:
+        chain.append(8)
+        n //= 8
+    chain.append(n)
+    if chain[-1] == 2 and len(chain) > 1:
+        chain[-2:] = [4, 4]
     return tuple(chain)
 
 
@@ -216,9 +231,10 @@ def entries() -> list[Entry]:
             add(_make(precision, r, 1, (r,)))
         for length in MIXED_ROWS:
             add(_make(precision, length, 1, chain_for(length)))
-        for length, chain in ALT_ROWS:
-            if length <= rows_max:
-                add(_make(precision, length, 1, chain))
+        for k in range(7, rows_max.bit_length()):
+            add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
+        for length, chain in ALT_RADIX4:
+            add(_make(precision, length, 1, chain))
         # strided kernels for multi-pass schedules
         for k in range(1, cols_max.bit_length()):
             length = 1 << k
@@ -227,7 +243,9 @@ def entries() -> list[Entry]:
             add(_make(precision, r, 0, (r,)))
         for length in MIXED_COLS:
             add(_make(precision, length, 0, chain_for(length)))
-        for length, chain in ALT_COLS:
+        for k in range(7, cols_max.bit_length()):
+            add(_make(precision, 1 << k, 0, _pow2_chain8(1 << k)))
+        for length, chain in ALT_RADIX4:
             add(_make(precision, length, 0, chain))
     return out
 
