@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--batch-log2", default="6,18",
                     help="c4: lengths 2^U{a..b}")
     ap.add_argument("--owned-tiles", type=int, default=0)
+    ap.add_argument("--batch-bits", type=int, default=32, choices=[32, 64])
     ap.add_argument("--item-tiles", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -300,9 +301,12 @@ def run_batch(args):
     lengths = [1 << int(k) for k in rng.integers(ka, kb + 1, count)]
     dirs = [mf.FORWARD if d else mf.BACKWARD for d in rng.integers(0, 2, count)]
     inplace = [bool(v) for v in rng.integers(0, 2, count)]
-    ins = [torch.view_as_complex(torch.randn(n, 2, device=dev, generator=g)) for n in lengths]
+    bits = args.batch_bits
+    fdt = torch.float32 if bits == 32 else torch.float64
+    ins = [torch.view_as_complex(torch.randn(n, 2, device=dev, generator=g, dtype=fdt))
+           for n in lengths]
     outs = [a if ip else torch.empty_like(a) for a, ip in zip(ins, inplace)]
-    bp = mf.BatchPlan(lengths, dirs, ins, outs, 32, graph=True,
+    bp = mf.BatchPlan(lengths, dirs, ins, outs, bits, graph=True,
                       owned_tiles=args.owned_tiles, item_tiles=args.item_tiles)
     d = bp.description()
 
@@ -327,7 +331,7 @@ def run_batch(args):
     # the same transforms, one single-transform plan execution each
     plans = {}
     for n, dr, ip in set(zip(lengths, dirs, inplace)):
-        plans[(n, dr, ip)] = mf.Plan(n, 1, 32, dr, in_place=ip)
+        plans[(n, dr, ip)] = mf.Plan(n, 1, bits, dr, in_place=ip)
 
     def each():
         for i, n in enumerate(lengths):
@@ -341,8 +345,8 @@ def run_batch(args):
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms, 4), "p50_ms": round(percentile(t, 0.5), 4),
         "p99_ms": round(percentile(t, 0.99), 4), "higher_is_better": True,
-        "dtype": "f32", "data": "synthetic (seeded on device)",
-        "config": {"workload": f"{count} fp32 transforms N=2^U{{{ka}..{kb}}} 50/50 fwd/bwd "
+        "dtype": f"f{bits}", "data": "synthetic (seeded on device)",
+        "config": {"workload": f"{count} fp{bits} transforms N=2^U{{{ka}..{kb}}} 50/50 fwd/bwd "
                                "mixed placement, one persistent launch (hipGraph)",
                    "jobs": count, "distinct_lengths": d.distinct_lengths,
                    "multi_pass_jobs": d.multi_pass_jobs, "shared_jobs": d.shared_jobs,

@@ -235,6 +235,10 @@ def entries() -> list[Entry]:
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
         for length, chain in ALT_RADIX4:
             add(_make(precision, length, 1, chain))
+        # the persistent batch's chains, so a single-transform plan can
+        # reproduce a batch transform exactly
+        for k in range(1, BATCH_ROWS_MAX.bit_length()):
+            add(_make(precision, 1 << k, 1, BATCH_CHAIN(precision, 1 << k)))
         # strided kernels for multi-pass schedules
         for k in range(1, cols_max.bit_length()):
             length = 1 << k
@@ -247,6 +251,8 @@ def entries() -> list[Entry]:
             add(_make(precision, 1 << k, 0, _pow2_chain8(1 << k)))
         for length, chain in ALT_RADIX4:
             add(_make(precision, length, 0, chain))
+        for k in range(1, BATCH_COLS_MAX.bit_length()):
+            add(_make(precision, 1 << k, 0, BATCH_CHAIN(precision, 1 << k)))
     return out
 
 
@@ -259,6 +265,22 @@ BATCH_ROWS_MAX = 4096
 BATCH_COLS_MAX = 512
 
 
+def _batch_chain(precision: int, length: int) -> tuple[int, ...]:
+    """FP32: the radix-16 chains (measured on 4096 transforms of N=2^6..2^18:
+    2.46 ms, against 2.57 ms with radix-8 chains - the kernel's registers
+    stay bound by its strided kinds, so the extra exchange is not repaid).
+    FP64: radix-8 chains from 64 points, (8, 4) and (4, 4) below, which take
+    the kernel from 187 to 141 VGPRs (radix-16 FP64 butterflies hold 64)."""
+    if precision == 32 or length < 16:
+        return _pow2_chain(length)
+    if length >= 64:
+        return _pow2_chain8(length)
+    return {32: (8, 4), 16: (4, 4)}[length]
+
+
+BATCH_CHAIN = _batch_chain
+
+
 def _pow2_floor(x: int) -> int:
     return 1 << (max(1, x).bit_length() - 1)
 
@@ -268,8 +290,8 @@ def batch_entries() -> list[Entry]:
     power-of-two entries, with threads per column a power of two and
     columns * threads = 256.  Strided kinds keep at least 16 (FP32) / 8
     (FP64) adjacent columns, i.e. 128-byte runs per access, unless that
-    leaves a thread more than 16 points of a sub-stage (register pressure
-    of the one kernel is that of its largest kind)."""
+    leaves a thread more than 16 (FP32) / 8 (FP64) points of a sub-stage
+    (register pressure of the one kernel is that of its largest kind)."""
     out: list[Entry] = []
 
     def points_per_thread(length, stages, tpt):
@@ -277,15 +299,17 @@ def batch_entries() -> list[Entry]:
 
     for precision in (32, 64):
         cmin = 16 if precision == 32 else 8
+        ppt_max = 16 if precision == 32 else 8
         for rows, lmax in ((1, BATCH_ROWS_MAX), (0, BATCH_COLS_MAX)):
             for k in range(1, lmax.bit_length()):
                 length = 1 << k
-                stages = _pow2_chain(length)
+                stages = BATCH_CHAIN(precision, length)
                 tpt = min(BATCH_THREADS, _pow2_floor(length // max(stages)))
                 if not rows:
                     tpt = min(tpt, BATCH_THREADS // cmin)
-                    while (points_per_thread(length, stages, tpt) > 16 and
-                           2 * tpt <= BATCH_THREADS):
+                    while (points_per_thread(length, stages, tpt) > ppt_max and
+                           2 * tpt <= BATCH_THREADS and
+                           (BATCH_THREADS // (2 * tpt)) * length % BATCH_THREADS == 0):
                         tpt *= 2
                 e = Entry(precision, length, rows, tpt, BATCH_THREADS // tpt, stages)
                 assert e.threads == BATCH_THREADS and e.lds_bytes <= LDS_CAP, e

@@ -174,7 +174,8 @@ static int find_kernel_variant(const mofft_kernel_set *ks, int precision_bits,
         memcmp(e->stages, stages, (size_t)count * sizeof(int)) == 0)
       return i;
   }
-  if (count == 1) {
+  /* radix 32 / 64 levels run as the two-stage L = 32 / 64 kernels */
+  if (count == 1 && (stages[0] == 32 || stages[0] == 64)) {
     for (int i = 0; i < *ks->count; ++i) {
       const mofft_registry_entry *e = &ks->entries[i];
       if (e->precision_bits == precision_bits && e->rows == rows &&
