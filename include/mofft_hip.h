@@ -173,7 +173,8 @@ int mofft_device_available(void);
  * strided passes up to 2^18, three beyond).  A multi-pass transform is run
  * whole by one workgroup (owned) unless it is larger than `owned_tiles`;
  * then its passes are split into items of `item_tiles` tiles spread over
- * workgroups that synchronise through device counters (shared).  The radix chain of each length
+ * workgroups that synchronise through device counters (shared).  The
+ * buffers of different transforms must not overlap.  The radix chain of each length
  * is the compiled wisdom's when its passes exist as batch tile kinds, else
  * the analytical planner's over those kinds. */
 typedef struct mofft_batch_plan mofft_batch_plan;
@@ -255,7 +256,8 @@ void mofft_batch_plan_destroy(mofft_batch_plan *plan);
  * their twiddles) stay resident per (length, precision, direction);
  * request buffers come from one device arena of `budget_bytes` allocated
  * at creation, so serving calls no device allocator.  Results are
- * out-of-place in server-owned buffers that stay valid until released. */
+ * out-of-place in server-owned buffers that stay valid until released.
+ * A server is driven from one host thread at a time. */
 typedef struct mofft_server mofft_server;
 
 typedef struct {

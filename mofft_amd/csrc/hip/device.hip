@@ -222,7 +222,7 @@ int mofft_hip_timer_end(void *stream, double *seconds) {
 
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream) {
-  if (kernel < 0 || kernel >= mofft::launch_table_size || !args)
+  if (kernel < 0 || kernel >= mofft::launch_table_size)
     return (int)hipErrorInvalidValue;
   return mofft::launch_table[kernel](args, stream);
 }

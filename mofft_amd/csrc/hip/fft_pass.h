@@ -534,6 +534,7 @@ struct PassKernel {
       }
       configured = true;
     }
+    if (!a) return 0; /* configure only (e.g. before a stream capture) */
     const int64_t blocks = (a->columns + C - 1) / C;
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;

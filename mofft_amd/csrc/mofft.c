@@ -2116,6 +2116,10 @@ static mofft_status graph_create(const mofft_plan_common *p, const void *in,
     return MOFFT_INVALID_ARGUMENT;
   if (!mofft_hip_is_device_pointer(in) || !mofft_hip_is_device_pointer(out))
     return MOFFT_INVALID_ARGUMENT;
+  /* kernel attributes are set outside the capture */
+  for (int i = 0; i < p->s.pass_count; ++i)
+    if (mofft_hip_launch_pass(p->s.passes[i].kernel, NULL, NULL) != 0)
+      return MOFFT_INTERNAL_ERROR;
   mofft_graph *g = (mofft_graph *)calloc(1, sizeof(*g));
   if (!g) return MOFFT_ALLOCATION_FAILURE;
   void *cs = NULL;

@@ -85,6 +85,8 @@ int mofft_hip_fill_stage(void *out, int precision_bits, int64_t ns, int r,
  * from FP64. */
 int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
                         void *stream);
+/* args == NULL only sets the kernel's launch attributes (large dynamic
+ * LDS), as the first launch would; nothing is enqueued. */
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream);
 /* Device-side interval timer on `stream` (HIP events): begin records the

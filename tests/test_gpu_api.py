@@ -225,3 +225,24 @@ def test_captured_graph_replays(gpu, torch_dev, bits, n, batch, in_place):
                 del junk
     ref = oracle.fft_dd(x, -1)
     assert oracle.rel_l2(want.cpu().numpy(), ref) <= tol(n, bits)
+
+
+def test_capture_before_first_launch(gpu, torch_dev):
+    """Capturing a plan whose kernel has never been launched in the process
+    (a 139 KiB-LDS matrix-core kernel: its launch attribute is set before the
+    capture) and replaying it equals a direct execution."""
+    torch = torch_dev
+    n, batch = 1 << 15, 4
+    x = seeded_complex((batch, n), seed=15, dtype=np.complex64)
+    src = to_dev(torch, x, 32)
+    out = torch.empty_like(src)
+    with gpu.Plan(n, batch, 32, gpu.FORWARD, mfma=True) as p:
+        with p.capture(src, out) as g:
+            g.launch(0)
+            torch.cuda.synchronize()
+            got = out.clone()
+        out.zero_()
+        p.execute_async(src, out, 0)
+        torch.cuda.synchronize()
+    assert torch.equal(got, out)
+    assert oracle.rel_l2(got.cpu().numpy(), oracle.fft_dd(x, -1)) <= tol(n, 32)
