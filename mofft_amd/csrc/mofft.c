@@ -41,12 +41,15 @@
 /* Device family token of recipes: the gfx target number in hex digits. */
 #define MOFFT_GPU_FAMILY 0x00000950u
 /* Multi-pass schedules stream the batch in chunks of at most this many bytes
- * (the workspace of in-place plans is one chunk).  Measured on MI355X: fewer,
- * larger launches beat Infinity-Cache-sized chunks (FP64 N=2^16: 4.9 vs 4.0
- * TFLOP/s at 256 vs 64 MiB) because each launch then fills all 256 CUs.
- * MOFFT_CHUNK_BYTES in the environment overrides it (a tuning knob; read
- * once per process). */
-#define MOFFT_DEFAULT_CHUNK_BYTES ((size_t)256 << 20)
+ * (the workspace of an in-place plan is one chunk).  128 MiB keeps a chunk
+ * and the intermediate its first pass writes inside the 256 MB Infinity
+ * Cache, so the second pass reads it from there; smaller chunks lose more
+ * to launch ramp-up than they gain.  Measured on MI355X, two-pass plans:
+ * FP64 N=2^16 x 256 fwd+bwd 5.57 TFLOP/s at 128 MiB vs 5.22 at 256, 5.29 at
+ * 64, 5.06 at 160; FP32 N=2^16..2^20 over 256 MiB 9-11% faster than at
+ * 256 MiB.  MOFFT_CHUNK_BYTES in the environment overrides it (a tuning
+ * knob; read once per process). */
+#define MOFFT_DEFAULT_CHUNK_BYTES ((size_t)128 << 20)
 /* Kernel bank of the matrix-core (split FP16) radix-16 kernels, and of
  * four-times-wider strided tiles. */
 #define MOFFT_MFMA_BANK 3u
