@@ -92,9 +92,9 @@ void mofft_batch_options_init(mofft_batch_options *options) {
 
 static int is_pow2(size_t n) { return n >= 2 && (n & (n - 1)) == 0; }
 
-/* The schedule one length uses in a batch: the wisdom chain when it maps
- * onto batch tile kinds in no more passes than the analytical chain over
- * those kinds, else the analytical chain. */
+/* The schedule one length uses in a batch: the wisdom chain when the batch
+ * tile kinds run it with the same passes as the tuned plan and no more than
+ * the analytical chain over those kinds, else the analytical chain. */
 static mofft_status batch_schedule(int precision_bits, size_t length,
                                    mofft_schedule *s, int *from_wisdom) {
   if (!is_pow2(length)) return MOFFT_UNSUPPORTED_SIZE;
@@ -109,13 +109,17 @@ static mofft_status batch_schedule(int precision_bits, size_t length,
   st = mofft_internal_build_schedule(&mofft_batch_kernel_set, length, 1,
                                      precision_bits, radices, depth, mask, 0, s);
   if (st != MOFFT_SUCCESS) return st;
-  mofft_schedule w;
+  /* the tuned chain, where the batch kinds reproduce its pass grouping */
+  mofft_schedule w, tuned;
   if (mofft_internal_wisdom_lookup(length, precision_bits, radices, &depth,
                                    &mask) > 0 &&
       mofft_internal_build_schedule(&mofft_batch_kernel_set, length, 1,
                                     precision_bits, radices, depth, mask, 0,
                                     &w) == MOFFT_SUCCESS &&
-      w.pass_count <= s->pass_count) {
+      mofft_internal_build_schedule(&mofft_pass_kernel_set, length, 1,
+                                    precision_bits, radices, depth, mask, 0,
+                                    &tuned) == MOFFT_SUCCESS &&
+      w.pass_count == tuned.pass_count && w.pass_count <= s->pass_count) {
     *s = w;
     *from_wisdom = 1;
   }

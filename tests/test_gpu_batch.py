@@ -198,3 +198,55 @@ def test_batch_rejects_host_pointers(gpu):
     with pytest.raises(gpu.MofftError) as e:
         gpu.BatchPlan([64], [-1], [x], [y], 32)
     assert e.value.status == gpu.INVALID_ARGUMENT
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_three_pass_and_edge_shapes(gpu, torch_dev, bits):
+    """Transforms of three strided passes (2^19..2^21), in and out of place,
+    next to the smallest lengths (2, 4), one of each, and a lone transform."""
+    torch = torch_dev
+    lengths = [1 << 19, 1 << 20, 1 << 21, 2, 4, 1 << 21, 8]
+    dirs = [-1, 1, -1, 1, -1, 1, -1]
+    inplace = [True, False, True, False, True, False, False]
+    rng = np.random.default_rng(bits)
+    xs, ins, outs = [], [], []
+    for n, ip in zip(lengths, inplace):
+        x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(cdtype(bits))
+        d = torch.from_numpy(x).to("cuda")
+        xs.append(x)
+        ins.append(d)
+        outs.append(d if ip else torch.empty_like(d))
+    for n in lengths[:3]:
+        assert gpu.batch_describe_length(n, bits).pass_count == 3
+    with gpu.BatchPlan(lengths, dirs, ins, outs, bits) as bp:
+        assert bp.description().multi_pass_jobs == 4
+        bp.execute()
+    torch.cuda.synchronize()
+    for i, n in enumerate(lengths):
+        ref = oracle.dft(xs[i].astype(np.complex128), dirs[i])
+        assert oracle.rel_l2(outs[i].cpu().numpy(), ref) <= tol(n, bits), (i, n)
+    # a lone two-element transform
+    x = np.array([1 + 2j, 3 - 1j], dtype=cdtype(bits))
+    d = torch.from_numpy(x).to("cuda")
+    with gpu.BatchPlan([2], [-1], [d], [d], bits) as bp:
+        bp.execute()
+    torch.cuda.synchronize()
+    assert np.allclose(d.cpu().numpy(), [4 + 1j, -2 + 3j])
+
+
+def test_many_equal_small_transforms(gpu, torch_dev):
+    """3000 transforms of one length: a single gathered segment."""
+    torch = torch_dev
+    n, count = 64, 3000
+    x = seeded_complex((count, n), seed=64, dtype=np.complex64)
+    d_in = torch.from_numpy(x).to("cuda")
+    d_out = torch.empty_like(d_in)
+    ins = [d_in[i] for i in range(count)]
+    outs = [d_out[i] for i in range(count)]
+    with gpu.BatchPlan([n] * count, [-1] * count, ins, outs, 32) as bp:
+        d = bp.description()
+        assert d.distinct_lengths == 1 and d.segment_count == 1 and d.multi_pass_jobs == 0
+        bp.execute()
+    torch.cuda.synchronize()
+    ref = oracle.dft(x.astype(np.complex128), -1)
+    assert oracle.rel_l2(d_out.cpu().numpy(), ref) <= tol(n, 32)

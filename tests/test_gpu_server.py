@@ -146,3 +146,46 @@ def test_poisson_trace(gpu):
     assert all(c.status == gpu.SUCCESS for c in cs)
     assert all(l > 0 for l in lat)
     assert sorted(c.tag for c in cs) == list(range(len(reqs)))
+
+
+def test_concurrent_three_pass_requests(gpu, hip_copy):
+    """Three-pass schedules need workspace: requests of one plan that run at
+    the same time on different streams each get their own from the arena."""
+    n, b = 1 << 21, 2
+    reqs = [(n, b, 32, -1 if i % 2 == 0 else 1, 1000 + i, i) for i in range(4)]
+    assert gpu.describe(n, b, 32).pass_count == 3
+    out = {}
+    with gpu.Server(1 << 30, streams=4, max_inflight=4) as srv:
+        for (nn, bb, bits, d, seed, tag) in reqs:
+            srv.submit(nn, bb, bits, d, seed=seed, tag=tag)
+        while len(out) < len(reqs):
+            for c in srv.poll(wait=True):
+                assert c.status == gpu.SUCCESS
+                out[c.tag] = hip_copy(c.output, n, b, 32)
+                srv.release(c.ticket)
+        assert srv.stats().peak_inflight == 4
+    for (nn, bb, bits, d, seed, tag) in reqs:
+        x = seeded_device_uniform(seed, n * b, 32).reshape(b, n).astype(np.complex128)
+        assert oracle.rel_l2(out[tag], oracle.dft(x, d)) <= tol(n, 32), tag
+
+
+def test_budget_serialises_large_requests(gpu, hip_copy):
+    """Requests that do not fit together wait for each other's memory; the
+    arena never holds more than the budget."""
+    n, b = 1 << 20, 8                  # 64 MiB in + 64 MiB out (FP64)
+    budget = 300 << 20                 # room for two at a time, not three
+    reqs = [(n, b, 64, -1, 7 + i, i) for i in range(5)]
+    done = {}
+    with gpu.Server(budget, streams=4, max_inflight=8, max_batch=8) as srv:
+        for (nn, bb, bits, d, seed, tag) in reqs:
+            srv.submit(nn, bb, bits, d, seed=seed, tag=tag)
+        assert srv.stats().inflight == 2 and srv.stats().pending == 3
+        while len(done) < len(reqs):
+            for c in srv.poll(wait=True):
+                assert c.status == gpu.SUCCESS
+                done[c.tag] = hip_copy(c.output, n, b, 64)
+                srv.release(c.ticket)
+        st = srv.stats()
+        assert st.peak_bytes_in_use <= budget and st.peak_inflight == 2
+    x = seeded_device_uniform(9, n * b, 64).reshape(b, n)
+    assert oracle.rel_l2(done[2], oracle.dft(x, -1)) <= tol(n, 64)
