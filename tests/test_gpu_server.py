@@ -172,8 +172,14 @@ def test_concurrent_three_pass_requests(gpu, hip_copy):
 def test_budget_serialises_large_requests(gpu, hip_copy):
     """Requests that do not fit together wait for each other's memory; the
     arena never holds more than the budget."""
-    n, b = 1 << 20, 8                  # 64 MiB in + 64 MiB out (FP64)
-    budget = 300 << 20                 # room for two at a time, not three
+    n, b = 1 << 20, 4                  # 64 MiB per FP64 buffer
+    with gpu.Server(1 << 30, max_batch=8) as srv:
+        srv.submit(n, b, 64, -1, seed=1)
+        per = srv.stats().bytes_in_use  # input + output + workspace
+        (c,) = srv.poll(wait=True)
+        srv.release(c.ticket)
+    assert per >= 2 * n * b * 16
+    budget = 2 * per + per // 2        # room for two at a time, not three
     reqs = [(n, b, 64, -1, 7 + i, i) for i in range(5)]
     done = {}
     with gpu.Server(budget, streams=4, max_inflight=8, max_batch=8) as srv:
