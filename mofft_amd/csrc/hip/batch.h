@@ -68,9 +68,9 @@ __device__ __forceinline__ void setup_gather(Ctx<T> &x,
   x.tid = tid;
   x.c = tid / TPT;
   x.t = tid % TPT;
-  x.col0 = block * C;
-  const int64_t col = x.col0 + x.c;
-  x.live = col < p.columns;
+  x.col0 = (uint32_t)block * C;
+  const uint32_t col = x.col0 + x.c;
+  x.live = col < (uint32_t)p.columns;
   const void *in = nullptr;
   void *out = nullptr;
   uint32_t flags = 0;

@@ -102,12 +102,14 @@ struct Ctx {
   const cx<T> *thi;
   const cx<T> *tlo;
   const cx<T> *tpass; /* &tw_pass[j % ns] when the full table is used */
-  int64_t stride;
-  int64_t ns;
-  int64_t obase;      /* (j / ns) * ns * L + j % ns */
-  int64_t col0;       /* first column of this tile */
-  uint64_t jm_scale;  /* (j % ns) * tw_scale */
-  uint64_t split_mask;
+  /* Element offsets are 32-bit: a launch touches fewer than
+   * MOFFT_PASS_MAX_ELEMENTS elements (checked on the host). */
+  uint32_t stride;
+  uint32_t ns;
+  uint32_t obase;     /* (j / ns) * ns * L + j % ns */
+  uint32_t col0;      /* first column of this tile */
+  uint32_t jm_scale;  /* (j % ns) * tw_scale */
+  uint32_t split_mask;
   uint32_t split_shift;
   int tid;            /* thread index in the workgroup */
   int c;
@@ -118,6 +120,16 @@ struct Ctx {
   bool pass_tw;
   bool pass_table;
 };
+
+/* x / d for a launch-invariant divisor d given by its multiplier and
+ * shift (mofft_div_magic in mofft_abi.h): a multiply-high, two adds and two
+ * shifts instead of a full integer division per lane. */
+__device__ __forceinline__ uint32_t div_by(uint32_t x, uint32_t magic,
+                                           uint32_t shift) {
+  if (shift == 0) return x; /* d == 1 */
+  const uint32_t t = __umulhi(x, magic);
+  return (t + ((x - t) >> 1)) >> (shift - 1);
+}
 
 /* Native clang vectors for the non-temporal load/store builtins. */
 typedef float nf2 __attribute__((ext_vector_type(2)));
@@ -167,10 +179,10 @@ __device__ __forceinline__ void pass_twiddle(const Ctx<T> &x, int e, T &a,
                                              T &b) {
   if (x.pass_table) {
     /* adjacent columns read adjacent entries: coalesced */
-    cmul<T>(a, b, x.tpass[(int64_t)e * x.ns]);
+    cmul<T>(a, b, x.tpass[(uint32_t)e * x.ns]);
     return;
   }
-  const uint64_t m = (uint64_t)e * x.jm_scale;
+  const uint32_t m = (uint32_t)e * x.jm_scale;
   const cx<T> h = x.thi[m >> x.split_shift];
   const cx<T> l = x.tlo[m & x.split_mask];
   cx<T> w;
@@ -182,7 +194,7 @@ __device__ __forceinline__ void pass_twiddle(const Ctx<T> &x, int e, T &a,
 template <typename T>
 __device__ __forceinline__ void load_global(const Ctx<T> &x, int e, T &a,
                                             T &b) {
-  const cx<T> v = ld_stream<T>(x.src + (int64_t)e * x.stride);
+  const cx<T> v = ld_stream<T>(x.src + (uint32_t)e * x.stride);
   a = x.swap_in ? v.y : v.x;
   b = x.swap_in ? v.x : v.y;
   if (x.pass_tw) pass_twiddle<T>(x, e, a, b);
@@ -207,23 +219,23 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
   constexpr int PER = (C * L) / TL::NT;
   static_assert((C * L) % TL::NT == 0, "every thread stores PER elements");
   const int tid = x.tid;
-  const int64_t ns = p.ns;
-  const int span = (int)ns * L;
+  const uint32_t ns = (uint32_t)p.ns;
+  const uint32_t span = ns * (uint32_t)L;
   int ad[PER];
 #pragma unroll
   for (int i = 0; i < PER; ++i) {
-    const int q = tid + i * TL::NT;
-    int cc, o;
+    const uint32_t q = (uint32_t)(tid + i * TL::NT);
+    uint32_t cc, o;
     if (ns == 1) {
       cc = q / L;
       o = q - cc * L;
     } else {
-      const int g = q / span;
-      const int rem = q - g * span;
-      o = rem / (int)ns;
-      cc = g * (int)ns + (rem - o * (int)ns);
+      const uint32_t g = div_by(q, p.span_magic, p.span_shift);
+      const uint32_t rem = q - g * span;
+      o = div_by(rem, p.ns_magic, p.ns_shift);
+      cc = g * ns + (rem - o * ns);
     }
-    ad[i] = TL::row_addr(cc, o);
+    ad[i] = TL::row_addr((int)cc, (int)o);
   }
   T outr[PER], outi[PER];
 #pragma unroll
@@ -246,36 +258,41 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
     }
   }
   cx<T> *out = (cx<T> *)p.out;
-  const int64_t col0 = x.col0;
-  const int64_t b0 = col0 / p.stride;
-  const int64_t j0 = col0 - b0 * p.stride;
+  const uint32_t col0 = x.col0;
+  const uint32_t stride = (uint32_t)p.stride;
+  const uint32_t n = (uint32_t)p.n;
+  const uint32_t b0 = div_by(col0, p.stride_magic, p.stride_shift);
+  const uint32_t j0 = col0 - b0 * stride;
+  const uint32_t jq0 = div_by(j0, p.ns_magic, p.ns_shift);
+  const uint32_t cq = div_by((uint32_t)C, p.ns_magic, p.ns_shift);
   const bool swap = (p.flags & MOFFT_PASS_SWAP_OUT) != 0;
-  const bool aligned = (j0 + C <= p.stride) && (C % ns == 0) &&
-                       (j0 % ns == 0) && (col0 + C <= p.columns);
+  const bool aligned = (j0 + C <= stride) && (cq * ns == (uint32_t)C) &&
+                       (jq0 * ns == j0) && (col0 + C <= (uint32_t)p.columns);
   if (aligned) {
-    cx<T> *base = out + b0 * p.n + (j0 / ns) * ns * L;
+    cx<T> *base = out + (b0 * n + jq0 * span);
 #pragma unroll
     for (int i = 0; i < PER; ++i)
       store_global<T>(base + tid + i * TL::NT, outr[i], outi[i], swap);
   } else {
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
-      const int q = tid + i * TL::NT;
-      int cc, o;
+      const uint32_t q = (uint32_t)(tid + i * TL::NT);
+      uint32_t cc, o;
       if (ns == 1) {
         cc = q / L;
         o = q - cc * L;
       } else {
-        const int g = q / span;
-        const int rem = q - g * span;
-        o = rem / (int)ns;
-        cc = g * (int)ns + (rem - o * (int)ns);
+        const uint32_t g = div_by(q, p.span_magic, p.span_shift);
+        const uint32_t rem = q - g * span;
+        o = div_by(rem, p.ns_magic, p.ns_shift);
+        cc = g * ns + (rem - o * ns);
       }
-      const int64_t col = col0 + cc;
-      if (col >= p.columns) continue;
-      const int64_t b = col / p.stride;
-      const int64_t j = col - b * p.stride;
-      const int64_t pos = b * p.n + (j / ns) * ns * L + (j % ns) + (int64_t)o * ns;
+      const uint32_t col = col0 + cc;
+      if (col >= (uint32_t)p.columns) continue;
+      const uint32_t b = div_by(col, p.stride_magic, p.stride_shift);
+      const uint32_t j = col - b * stride;
+      const uint32_t jq = div_by(j, p.ns_magic, p.ns_shift);
+      const uint32_t pos = b * n + jq * span + (j - jq * ns) + o * ns;
       store_global<T>(out + pos, outr[i], outi[i], swap);
     }
   }
@@ -405,8 +422,8 @@ struct Stage {
         if (FULL || jj < NB) {
 #pragma unroll
           for (int k = 0; k < R; ++k) {
-            const int64_t o = jj + k * NB;
-            cx<T> *ptr = ROWS ? x.dst + o : x.dst + x.obase + o * x.ns;
+            const int o = jj + k * NB;
+            cx<T> *ptr = ROWS ? x.dst + o : x.dst + (x.obase + (uint32_t)o * x.ns);
             store_global<T>(ptr, vr[it][k], vi[it][k], x.swap_out);
           }
         }
@@ -469,24 +486,24 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.tid = tid;
   x.c = ROWS ? tid / TPT : tid % C;
   x.t = ROWS ? tid % TPT : tid / C;
-  x.col0 = block * C;
-  const int64_t col = x.col0 + x.c;
-  x.live = col < p.columns;
-  const int64_t b = ROWS ? col : col / p.stride;
-  const int64_t j = ROWS ? 0 : col - b * p.stride;
-  x.stride = p.stride;
-  x.ns = p.ns;
-  x.src = (const cx<T> *)p.in + b * p.n + j;
-  x.dst = (cx<T> *)p.out + b * p.n;
+  x.col0 = (uint32_t)block * C;
+  const uint32_t col = x.col0 + x.c;
+  x.live = col < (uint32_t)p.columns;
+  const uint32_t b = ROWS ? col : div_by(col, p.stride_magic, p.stride_shift);
+  const uint32_t j = ROWS ? 0u : col - b * (uint32_t)p.stride;
+  x.stride = (uint32_t)p.stride;
+  x.ns = (uint32_t)p.ns;
+  x.src = (const cx<T> *)p.in + (b * (uint32_t)p.n + j);
+  x.dst = (cx<T> *)p.out + b * (uint32_t)p.n;
   x.stw = (const cx<T> *)p.stage_tw;
   x.thi = (const cx<T> *)p.tw_hi;
   x.tlo = (const cx<T> *)p.tw_lo;
-  const int64_t jq = p.ns > 1 ? j / p.ns : j;
-  const int64_t jr = j - jq * p.ns;
-  x.obase = jq * p.ns * L + jr;
-  x.jm_scale = (uint64_t)jr * (uint64_t)p.tw_scale;
+  const uint32_t jq = div_by(j, p.ns_magic, p.ns_shift);
+  const uint32_t jr = j - jq * x.ns;
+  x.obase = jq * x.ns * L + jr;
+  x.jm_scale = jr * (uint32_t)p.tw_scale;
   x.split_shift = p.split_shift;
-  x.split_mask = (uint64_t(1) << p.split_shift) - 1;
+  x.split_mask = (1u << p.split_shift) - 1u;
   x.swap_in = (p.flags & MOFFT_PASS_SWAP_IN) != 0;
   x.swap_out = (p.flags & MOFFT_PASS_SWAP_OUT) != 0;
   x.pass_tw = (p.flags & MOFFT_PASS_TWIDDLE) != 0 && jr != 0;

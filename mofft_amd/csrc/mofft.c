@@ -262,6 +262,8 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   if (depth < 64 && depth > 0 && (mask >> (depth - 1)) != 0)
     return MOFFT_INVALID_ARGUMENT;
   if (bank >= kernel_bank_count()) return MOFFT_INVALID_ARGUMENT;
+  /* element offsets inside a launch are 32-bit */
+  if (length > (size_t)MOFFT_PASS_MAX_ELEMENTS / 2) return MOFFT_UNSUPPORTED_SIZE;
   memset(s, 0, sizeof(*s));
   s->ks = ks;
   s->length = length;
@@ -353,6 +355,9 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   if (chunk > batch) chunk = batch;
   if (chunk == 0) chunk = 1;
   s->chunk = s->pass_count > 1 ? chunk : batch;
+  /* ... so one launch covers fewer than MOFFT_PASS_MAX_ELEMENTS / 2 */
+  const size_t max_chunk = (size_t)(MOFFT_PASS_MAX_ELEMENTS / 2) / length;
+  if (s->chunk > max_chunk) s->chunk = max_chunk;
   s->workspace_bytes = s->pass_count > 1 ? sat_mul(s->chunk, transform_bytes) : 0;
   return MOFFT_SUCCESS;
 }
@@ -675,6 +680,7 @@ static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
       a.tw_scale = (int64_t)(s->length / (ps->span * ps->length));
       a.columns = (int64_t)(cnt * ps->stride);
       a.split_shift = s->split_shift;
+      mofft_pass_args_set_divisors(&a, (int64_t)ps->length);
       a.flags = 0;
       if (backward && i == 0) a.flags |= MOFFT_PASS_SWAP_IN;
       if (backward && i == P - 1) a.flags |= MOFFT_PASS_SWAP_OUT;

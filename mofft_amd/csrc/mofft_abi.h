@@ -40,7 +40,45 @@ typedef struct {
   int64_t columns;        /* transforms * stride */
   uint32_t split_shift;
   uint32_t flags;         /* MOFFT_PASS_* */
+  /* Division by the launch-invariant divisors stride, ns and span = ns * L
+   * as a multiply-high (Granlund and Montgomery, "Division by invariant
+   * integers using multiplication", PLDI 1994, fig. 4.1): for a divisor
+   * d >= 2, shift = ceil(log2 d) and magic = floor(2^32 (2^shift - d) / d)
+   * + 1; then for every 32-bit x, t = mulhi(x, magic) and
+   * x / d = (t + ((x - t) >> 1)) >> (shift - 1).  d = 1 is magic = shift =
+   * 0.  Filled by mofft_pass_args_set_divisors(). */
+  uint32_t stride_magic;
+  uint32_t stride_shift;
+  uint32_t ns_magic;
+  uint32_t ns_shift;
+  uint32_t span_magic;
+  uint32_t span_shift;
 } mofft_pass_args;
+
+/* Element offsets inside one launch are 32-bit: columns * L (the elements
+ * a launch touches) and n stay below this bound. */
+#define MOFFT_PASS_MAX_ELEMENTS ((int64_t)1 << 31)
+
+static inline void mofft_div_magic(uint64_t d, uint32_t *magic,
+                                   uint32_t *shift) {
+  uint32_t l = 0;
+  if (d <= 1) {
+    *magic = 0;
+    *shift = 0;
+    return;
+  }
+  while (((uint64_t)1 << l) < d) ++l;
+  *magic = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+  *shift = l;
+}
+
+/* Fill the divisor fields from stride, ns and the pass length L. */
+static inline void mofft_pass_args_set_divisors(mofft_pass_args *a,
+                                                int64_t L) {
+  mofft_div_magic((uint64_t)a->stride, &a->stride_magic, &a->stride_shift);
+  mofft_div_magic((uint64_t)a->ns, &a->ns_magic, &a->ns_shift);
+  mofft_div_magic((uint64_t)(a->ns * L), &a->span_magic, &a->span_shift);
+}
 
 #define MOFFT_PASS_SWAP_IN 1u   /* exchange re/im on load (backward) */
 #define MOFFT_PASS_SWAP_OUT 2u  /* exchange re/im on store (backward) */

@@ -192,6 +192,7 @@ static void pass_args(const mofft_batch_plan *p, size_t t, const void *in,
   a->tw_scale = (int64_t)(s->length / (ps->span * ps->length));
   a->columns = (int64_t)ps->stride;
   a->split_shift = s->split_shift;
+  mofft_pass_args_set_divisors(a, (int64_t)ps->length);
   const int backward = p->directions[t] == MOFFT_BACKWARD;
   if (backward && i == 0) a->flags |= MOFFT_PASS_SWAP_IN;
   if (backward && i == P - 1) a->flags |= MOFFT_PASS_SWAP_OUT;
@@ -383,6 +384,7 @@ static mofft_status build_queue(mofft_batch_plan *p, const size_t *order,
     sg.args.ns = 1;
     sg.args.tw_scale = 1;
     sg.args.columns = (int64_t)b->gather_count;
+    mofft_pass_args_set_divisors(&sg.args, (int64_t)b->length);
     sg.jobs = (const mofft_batch_job *)p->d_jobs + b->gather_first;
     sg.tiles = (uint32_t)(((int64_t)b->gather_count + e->columns_per_block - 1) /
                           e->columns_per_block);

@@ -340,3 +340,62 @@ def test_device_generator_restatement_properties():
     z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
     z ^= z >> 31
     assert v[0] == 2.0 * ((z >> 11) * 2.0 ** -53) - 1.0
+
+
+DIV_CHECK = r"""
+#include <stdio.h>
+#include <stdint.h>
+#include "mofft_abi.h"
+/* the device's div_by, compiled for the host */
+static uint32_t div_by(uint32_t x, uint32_t m, uint32_t s) {
+  if (s == 0) return x;
+  uint32_t t = (uint32_t)(((uint64_t)x * m) >> 32);
+  return (t + ((x - t) >> 1)) >> (s - 1);
+}
+static uint64_t rng = 0x9e3779b97f4a7c15ull;
+static uint32_t next(void) { rng ^= rng << 13; rng ^= rng >> 7; rng ^= rng << 17; return (uint32_t)rng; }
+int main(void) {
+  uint32_t ds[6000]; int nd = 0;
+  for (uint32_t d = 1; d <= 4200; ++d) ds[nd++] = d;
+  for (int k = 12; k <= 30; ++k) { ds[nd++] = 1u << k; ds[nd++] = (1u << k) - 1; ds[nd++] = (1u << k) + 1; }
+  const uint32_t odd[] = {371293u, 759375u, 1594323u, 2147483647u, 2147483629u, 1073741827u, 3u * 5u * 7u * 11u * 13u * 13u};
+  for (unsigned i = 0; i < sizeof(odd) / sizeof(odd[0]); ++i) ds[nd++] = odd[i];
+  while (nd < 6000) ds[nd++] = 1u + (next() >> (next() % 31));
+  long bad = 0;
+  for (int i = 0; i < nd; ++i) {
+    const uint32_t d = ds[i];
+    uint32_t m, s;
+    mofft_div_magic(d, &m, &s);
+    uint32_t xs[64]; int nx = 0;
+    const uint32_t edge[] = {0u, 1u, d - 1u, d, d + 1u, 2u * d - 1u, 2u * d, 0x7fffffffu, 0x80000000u, 0xffffffffu, 0xfffffffeu};
+    for (unsigned k = 0; k < sizeof(edge) / sizeof(edge[0]); ++k) xs[nx++] = edge[k];
+    const uint32_t top = 0xffffffffu / d;
+    xs[nx++] = top * d; xs[nx++] = top * d - 1u; xs[nx++] = top * d + (d - 1u);
+    while (nx < 64) xs[nx++] = next();
+    for (int k = 0; k < nx; ++k)
+      if (div_by(xs[k], m, s) != xs[k] / d) { if (bad < 5) printf("d=%u x=%u\n", d, xs[k]); ++bad; }
+    for (uint32_t x = 0; x < 20000u; ++x)
+      if (div_by(x, m, s) != x / d) { if (bad < 5) printf("d=%u x=%u\n", d, x); ++bad; }
+  }
+  printf("bad %ld\n", bad);
+  return bad != 0;
+}
+"""
+
+
+def test_invariant_divisor_multipliers(tmp_path):
+    """The multiply-high division of the pass kernels (mofft_div_magic and
+    the device's div_by) equals integer division for every divisor class a
+    launch uses (1, small, powers of two and their neighbours, the odd
+    mixed-radix lengths, primes near 2^31) at edge and random dividends
+    across the whole 32-bit range."""
+    import subprocess
+    src = tmp_path / "div.c"
+    src.write_text(DIV_CHECK)
+    exe = tmp_path / "div"
+    csrc = Path(mf.__file__).parent / "csrc"
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-I", str(csrc), str(src), "-o", str(exe)],
+                   check=True)
+    res = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stdout
+    assert res.stdout.strip() == "bad 0"
