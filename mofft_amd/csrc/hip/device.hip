@@ -17,6 +17,11 @@ namespace mofft {
 typedef int (*launch_fn)(const mofft_pass_args *, void *);
 extern const launch_fn launch_table[];
 extern const int launch_table_size;
+/* XCD-resident two-pass launchers (nullptr where a kernel has none) */
+typedef int (*fused_fn)(const mofft_fused_args *, int, void *);
+typedef int (*fused_occupancy_fn)(int *, int *);
+extern const fused_fn fused_table[];
+extern const fused_occupancy_fn fused_occupancy_table[];
 /* persistent batch kernels (generated mofft_batch_f32/f64.hip) */
 int batch_occupancy_f32(int lds, int *bpc, int *cus);
 int batch_occupancy_f64(int lds, int *bpc, int *cus);
@@ -227,6 +232,23 @@ int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
   return mofft::launch_table[kernel](args, stream);
 }
 
+int mofft_hip_fused_available(int kernel) {
+  return kernel >= 0 && kernel < mofft::launch_table_size &&
+         mofft::fused_table[kernel] != nullptr;
+}
+
+int mofft_hip_fused_occupancy(int kernel, int *blocks_per_cu, int *cu_count) {
+  if (!mofft_hip_fused_available(kernel) || !blocks_per_cu || !cu_count)
+    return (int)hipErrorInvalidValue;
+  return mofft::fused_occupancy_table[kernel](blocks_per_cu, cu_count);
+}
+
+int mofft_hip_fused_launch(int kernel, const mofft_fused_args *args, int grid,
+                           void *stream) {
+  if (!mofft_hip_fused_available(kernel)) return (int)hipErrorInvalidValue;
+  return mofft::fused_table[kernel](args, grid, stream);
+}
+
 int mofft_hip_batch_occupancy(int precision_bits, int lds_bytes,
                               int *blocks_per_cu, int *cu_count) {
   if (!blocks_per_cu || !cu_count || lds_bytes < 0)
@@ -358,6 +380,10 @@ int mofft_hip_event_destroy(void *event) {
 
 int mofft_hip_event_record(void *event, void *stream) {
   return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
+}
+
+int mofft_hip_stream_wait_event(void *stream, void *event) {
+  return (int)hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0);
 }
 
 int mofft_hip_event_query(void *event) {

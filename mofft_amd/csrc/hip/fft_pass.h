@@ -115,6 +115,8 @@ struct Ctx {
   int c;
   int t;
   bool live;
+  bool coherent;      /* input written in this launch by another CU of the
+                         same XCD: read it from the L2, not the CU's L1 */
   bool swap_in;
   bool swap_out;
   bool pass_tw;
@@ -174,30 +176,29 @@ __device__ __forceinline__ void cmul(T &re, T &im, const cx<T> w) {
   im = i;
 }
 
+/* Element load that misses the CU's L1 and reads the L2 (relaxed
+ * agent-scope atomic loads of the 8-byte words). */
 template <typename T>
-__device__ __forceinline__ void pass_twiddle(const Ctx<T> &x, int e, T &a,
-                                             T &b) {
-  if (x.pass_table) {
-    /* adjacent columns read adjacent entries: coalesced */
-    cmul<T>(a, b, x.tpass[(uint32_t)e * x.ns]);
-    return;
+__device__ __forceinline__ cx<T> ld_l2(const cx<T> *p) {
+  const unsigned long long *w = reinterpret_cast<const unsigned long long *>(p);
+  cx<T> r;
+  if constexpr (sizeof(T) == 4) {
+    const unsigned long long v = __hip_atomic_load(
+        const_cast<unsigned long long *>(w), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    r.x = __uint_as_float((unsigned)v);
+    r.y = __uint_as_float((unsigned)(v >> 32));
+  } else {
+    const unsigned long long a = __hip_atomic_load(
+        const_cast<unsigned long long *>(w), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load(
+        const_cast<unsigned long long *>(w + 1), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT);
+    r.x = __longlong_as_double((long long)a);
+    r.y = __longlong_as_double((long long)b);
   }
-  const uint32_t m = (uint32_t)e * x.jm_scale;
-  const cx<T> h = x.thi[m >> x.split_shift];
-  const cx<T> l = x.tlo[m & x.split_mask];
-  cx<T> w;
-  w.x = h.x * l.x - h.y * l.y;
-  w.y = h.x * l.y + h.y * l.x;
-  cmul<T>(a, b, w);
-}
-
-template <typename T>
-__device__ __forceinline__ void load_global(const Ctx<T> &x, int e, T &a,
-                                            T &b) {
-  const cx<T> v = ld_stream<T>(x.src + (uint32_t)e * x.stride);
-  a = x.swap_in ? v.y : v.x;
-  b = x.swap_in ? v.x : v.y;
-  if (x.pass_tw) pass_twiddle<T>(x, e, a, b);
+  return r;
 }
 
 template <typename T>
@@ -331,7 +332,31 @@ struct Stage {
   static constexpr bool FULL = IT * TPT == NB;
   static constexpr bool LAST = sizeof...(REST) == 0;
 
-  /* first sub-stage: element e = jj + k*NB straight from HBM */
+  /* first sub-stage: element e = jj + k*NB straight from HBM.  Every
+   * load (data, then pass twiddles) is issued before any result is used:
+   * a use between two loads makes the compiler wait for the first before
+   * issuing the second, one memory round trip per element. */
+  template <bool L2>
+  static __device__ __forceinline__ void load_raw(const Ctx<T> &x,
+                                                  cx<T> (&raw)[IT][R]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int jj = x.t + it * TPT;
+      const bool ok = (FULL || jj < NB) && x.live;
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        cx<T> v;
+        v.x = T(0);
+        v.y = T(0);
+        if (ok) {
+          const cx<T> *q = x.src + (uint32_t)(jj + k * NB) * x.stride;
+          v = L2 ? ld_l2<T>(q) : ld_stream<T>(q);
+        }
+        raw[it][k] = v;
+      }
+    }
+  }
+
   static __device__ __forceinline__ void load_first(const Ctx<T> &x,
                                                     T (&vr)[IT][R],
                                                     T (&vi)[IT][R]) {
@@ -339,31 +364,86 @@ struct Stage {
                   (R % 2 == 0) && sizeof(T) == 4) {
       /* a whole small transform per thread: 16-byte loads */
       const nf4 *s = reinterpret_cast<const nf4 *>(x.src);
+      nf4 v[R / 2];
+#pragma unroll
+      for (int k = 0; k < R / 2; ++k) {
+        v[k] = nf4{0.f, 0.f, 0.f, 0.f};
+#if MOFFT_NT_LOAD
+        if (x.live) v[k] = __builtin_nontemporal_load(s + k);
+#else
+        if (x.live) v[k] = s[k];
+#endif
+      }
 #pragma unroll
       for (int k = 0; k < R; k += 2) {
-        nf4 v = {0.f, 0.f, 0.f, 0.f};
-#if MOFFT_NT_LOAD
-        if (x.live) v = __builtin_nontemporal_load(s + k / 2);
-#else
-        if (x.live) v = s[k / 2];
-#endif
-        vr[0][k] = x.swap_in ? v.y : v.x;
-        vi[0][k] = x.swap_in ? v.x : v.y;
-        vr[0][k + 1] = x.swap_in ? v.w : v.z;
-        vi[0][k + 1] = x.swap_in ? v.z : v.w;
+        const nf4 w = v[k / 2];
+        vr[0][k] = x.swap_in ? w.y : w.x;
+        vi[0][k] = x.swap_in ? w.x : w.y;
+        vr[0][k + 1] = x.swap_in ? w.w : w.z;
+        vi[0][k + 1] = x.swap_in ? w.z : w.w;
       }
     } else {
+      cx<T> raw[IT][R];
+      if (x.coherent) load_raw<true>(x, raw);
+      else load_raw<false>(x, raw);
+      /* w_N^(e * (j % ns) * tw_scale) per element (1 where unused) */
+      cx<T> w[IT][R];
 #pragma unroll
-      for (int it = 0; it < IT; ++it) {
-        const int jj = x.t + it * TPT;
-        const bool ok = (FULL || jj < NB) && x.live;
+      for (int it = 0; it < IT; ++it)
 #pragma unroll
         for (int k = 0; k < R; ++k) {
-          T a = T(0), b = T(0);
-          if (ok) load_global<T>(x, jj + k * NB, a, b);
-          vr[it][k] = a;
-          vi[it][k] = b;
+          w[it][k].x = T(1);
+          w[it][k].y = T(0);
         }
+      if (x.pass_tw) {
+        if (x.pass_table) {
+#pragma unroll
+          for (int it = 0; it < IT; ++it) {
+            const int jj = x.t + it * TPT;
+            if (FULL || jj < NB) {
+#pragma unroll
+              for (int k = 0; k < R; ++k)
+                w[it][k] = x.tpass[(uint32_t)(jj + k * NB) * x.ns];
+            }
+          }
+        } else {
+          cx<T> h[IT][R], l[IT][R];
+#pragma unroll
+          for (int it = 0; it < IT; ++it) {
+            const int jj = x.t + it * TPT;
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+              h[it][k] = w[it][k];
+              l[it][k] = w[it][k];
+              if (FULL || jj < NB) {
+                const uint32_t m = (uint32_t)(jj + k * NB) * x.jm_scale;
+                h[it][k] = x.thi[m >> x.split_shift];
+                l[it][k] = x.tlo[m & x.split_mask];
+              }
+            }
+          }
+#pragma unroll
+          for (int it = 0; it < IT; ++it)
+#pragma unroll
+            for (int k = 0; k < R; ++k) {
+              const cx<T> a = h[it][k], b = l[it][k];
+              w[it][k].x = a.x * b.x - a.y * b.y;
+              w[it][k].y = a.x * b.y + a.y * b.x;
+            }
+        }
+      }
+#pragma unroll
+      for (int it = 0; it < IT; ++it)
+#pragma unroll
+        for (int k = 0; k < R; ++k) {
+          vr[it][k] = x.swap_in ? raw[it][k].y : raw[it][k].x;
+          vi[it][k] = x.swap_in ? raw[it][k].x : raw[it][k].y;
+        }
+      if (x.pass_tw) {
+#pragma unroll
+        for (int it = 0; it < IT; ++it)
+#pragma unroll
+          for (int k = 0; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], w[it][k]);
       }
     }
   }
@@ -509,6 +589,7 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.pass_tw = (p.flags & MOFFT_PASS_TWIDDLE) != 0 && jr != 0;
   x.pass_table = (p.flags & MOFFT_PASS_TWIDDLE_TABLE) != 0;
   x.tpass = (const cx<T> *)p.tw_pass + jr;
+  x.coherent = false;
 }
 
 /* Load, transform and store the tile described by x. */

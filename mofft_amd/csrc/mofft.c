@@ -101,6 +101,20 @@ typedef struct mofft_plan_common {
   unsigned int kernel_bank;
   void *twiddles;
   void *workspace;
+  /* second stream of the chunk pipeline (odd chunks), its workspace and
+   * the fork/join events; pipe_state 0 = not set up, 1 = ready, -1 =
+   * unavailable */
+  void *pipe_stream;
+  void *pipe_workspace;
+  void *pipe_fork;
+  void *pipe_join;
+  int pipe_state;
+  /* XCD-resident two-pass execution: team workspace slots followed by the
+   * counters; fused_state 0 = not set up, 1 = ready, -1 = not used */
+  void *fused_work;
+  size_t fused_slot_bytes;
+  int fused_grid;
+  int fused_state;
   void *staging_in;
   void *staging_out;
   const void *bound_input;
@@ -532,8 +546,25 @@ const char *mofft_wisdom_source(void) { return mofft_generated_wisdom_source(); 
 /* plan lifetime                                                        */
 /* ------------------------------------------------------------------ */
 
+static void pipe_release(mofft_plan_common *p) {
+  if (p->pipe_stream) mofft_hip_stream_destroy(p->pipe_stream);
+  if (p->pipe_fork) mofft_hip_event_destroy(p->pipe_fork);
+  if (p->pipe_join) mofft_hip_event_destroy(p->pipe_join);
+  mofft_hip_free(p->pipe_workspace);
+  p->pipe_stream = p->pipe_fork = p->pipe_join = p->pipe_workspace = NULL;
+  p->pipe_state = 0;
+}
+
+static void fused_release(mofft_plan_common *p) {
+  mofft_hip_free(p->fused_work);
+  p->fused_work = NULL;
+  p->fused_state = 0;
+}
+
 static void plan_release(mofft_plan_common *p) {
   if (!p) return;
+  pipe_release(p);
+  fused_release(p);
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
   mofft_hip_free(p->staging_in);
@@ -642,64 +673,224 @@ static int launch_schedule_count(const mofft_plan_common *p, const void *in,
   return launch_schedule_timed(p, in, out, work, stream, count, NULL);
 }
 
-static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
-                                 void *out, void *work, void *stream,
-                                 size_t count, launch_timing *timing) {
+/* Arguments of pass i over `cnt` transforms from src to dst. */
+static void fill_pass_args(const mofft_plan_common *p, int i, const void *src,
+                           void *dst, size_t cnt, mofft_pass_args *a) {
   const mofft_schedule *s = &p->s;
   const size_t cb = complex_bytes(s->precision_bits);
   const char *tw = (const char *)p->twiddles;
   const int backward = p->direction == MOFFT_BACKWARD;
   const int P = s->pass_count;
+  const mofft_pass *ps = &s->passes[i];
+  memset(a, 0, sizeof(*a));
+  a->in = src;
+  a->out = dst;
+  a->stage_tw = tw ? tw + ps->stage_offset * cb : NULL;
+  a->tw_hi = tw ? tw + s->tw_hi_offset * cb : NULL;
+  a->tw_lo = tw ? tw + s->tw_lo_offset * cb : NULL;
+  a->tw_pass = (tw && ps->pass_table) ? tw + ps->pass_offset * cb : NULL;
+  a->n = (int64_t)s->length;
+  a->stride = (int64_t)ps->stride;
+  a->ns = (int64_t)ps->span;
+  a->tw_scale = (int64_t)(s->length / (ps->span * ps->length));
+  a->columns = (int64_t)(cnt * ps->stride);
+  a->split_shift = s->split_shift;
+  mofft_pass_args_set_divisors(a, (int64_t)ps->length);
+  a->flags = 0;
+  if (backward && i == 0) a->flags |= MOFFT_PASS_SWAP_IN;
+  if (backward && i == P - 1) a->flags |= MOFFT_PASS_SWAP_OUT;
+  if (ps->span > 1) a->flags |= MOFFT_PASS_TWIDDLE;
+  if (ps->span > 1 && ps->pass_table) a->flags |= MOFFT_PASS_TWIDDLE_TABLE;
+}
+
+/* The passes of one chunk of `cnt` transforms starting at transform c0. */
+static int launch_chunk(const mofft_plan_common *p, const void *in, void *out,
+                        void *work, void *stream, size_t c0, size_t cnt,
+                        launch_timing *timing) {
+  const mofft_schedule *s = &p->s;
+  const size_t cb = complex_bytes(s->precision_bits);
+  const int P = s->pass_count;
   const int in_place = in == out;
+  const char *src_chunk = (const char *)in + c0 * s->length * cb;
+  char *dst_chunk = (char *)out + c0 * s->length * cb;
+  const void *src = src_chunk;
+  for (int i = 0; i < P; ++i) {
+    void *dst;
+    if (i == P - 1) {
+      dst = dst_chunk;
+    } else if (in_place) {
+      dst = (i % 2 == 0) ? work : dst_chunk;
+    } else {
+      dst = ((P - 2 - i) % 2 == 0) ? (void *)dst_chunk : work;
+    }
+    const mofft_pass *ps = &s->passes[i];
+    mofft_pass_args a;
+    fill_pass_args(p, i, src, dst, cnt, &a);
+    if (timing && timing->used == 0) {
+      if (timing->capacity < 1) return 1;
+      if (mofft_hip_event_record(timing->events[0], stream) != 0) return 1;
+    }
+    const int rc = mofft_hip_launch_pass(ps->kernel, &a, stream);
+    if (rc != 0) return rc;
+    if (timing) {
+      if (timing->used + 2 > timing->capacity) return 1;
+      if (mofft_hip_event_record(timing->events[++timing->used], stream) != 0)
+        return 1;
+    }
+    src = dst;
+  }
+  return 0;
+}
+
+static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
+                                 void *out, void *work, void *stream,
+                                 size_t count, launch_timing *timing) {
+  const mofft_schedule *s = &p->s;
   for (size_t c0 = 0; c0 < count; c0 += s->chunk) {
     const size_t cnt = count - c0 < s->chunk ? count - c0 : s->chunk;
-    const char *src_chunk = (const char *)in + c0 * s->length * cb;
-    char *dst_chunk = (char *)out + c0 * s->length * cb;
-    const void *src = src_chunk;
-    for (int i = 0; i < P; ++i) {
-      void *dst;
-      if (i == P - 1) {
-        dst = dst_chunk;
-      } else if (in_place) {
-        dst = (i % 2 == 0) ? work : dst_chunk;
-      } else {
-        dst = ((P - 2 - i) % 2 == 0) ? (void *)dst_chunk : work;
-      }
-      const mofft_pass *ps = &s->passes[i];
-      mofft_pass_args a;
-      memset(&a, 0, sizeof(a));
-      a.in = src;
-      a.out = dst;
-      a.stage_tw = tw ? tw + ps->stage_offset * cb : NULL;
-      a.tw_hi = tw ? tw + s->tw_hi_offset * cb : NULL;
-      a.tw_lo = tw ? tw + s->tw_lo_offset * cb : NULL;
-      a.tw_pass = (tw && ps->pass_table) ? tw + ps->pass_offset * cb : NULL;
-      a.n = (int64_t)s->length;
-      a.stride = (int64_t)ps->stride;
-      a.ns = (int64_t)ps->span;
-      a.tw_scale = (int64_t)(s->length / (ps->span * ps->length));
-      a.columns = (int64_t)(cnt * ps->stride);
-      a.split_shift = s->split_shift;
-      mofft_pass_args_set_divisors(&a, (int64_t)ps->length);
-      a.flags = 0;
-      if (backward && i == 0) a.flags |= MOFFT_PASS_SWAP_IN;
-      if (backward && i == P - 1) a.flags |= MOFFT_PASS_SWAP_OUT;
-      if (ps->span > 1) a.flags |= MOFFT_PASS_TWIDDLE;
-      if (ps->span > 1 && ps->pass_table) a.flags |= MOFFT_PASS_TWIDDLE_TABLE;
-      if (timing && timing->used == 0) {
-        if (timing->capacity < 1) return 1;
-        if (mofft_hip_event_record(timing->events[0], stream) != 0) return 1;
-      }
-      const int rc = mofft_hip_launch_pass(ps->kernel, &a, stream);
-      if (rc != 0) return rc;
-      if (timing) {
-        if (timing->used + 2 > timing->capacity) return 1;
-        if (mofft_hip_event_record(timing->events[++timing->used], stream) != 0)
-          return 1;
-      }
-      src = dst;
-    }
+    const int rc = launch_chunk(p, in, out, work, stream, c0, cnt, timing);
+    if (rc != 0) return rc;
   }
+  return 0;
+}
+
+/* XCD-resident two-pass execution (mofft_abi.h, hip/fused.h): teams of
+ * workgroups, one per XCD, each with FUSED_SLOTS workspace slots. */
+#define FUSED_TEAMS 8u
+#define FUSED_SLOTS 3u
+/* Largest transform (bytes) run fused: the slots a team has in flight stay
+ * within its XCD's 4 MB L2.  MOFFT_FUSED_BYTES overrides it (0 = never). */
+#define FUSED_MAX_BYTES ((size_t)1 << 20)
+#define FUSED_MIN_BATCH 16u
+
+static size_t fused_max_bytes(void) {
+  static int cached = 0;
+  static size_t v = FUSED_MAX_BYTES;
+  if (!cached) {
+    const char *env = getenv("MOFFT_FUSED_BYTES");
+    if (env && *env) {
+      char *end = NULL;
+      const unsigned long long x = strtoull(env, &end, 10);
+      if (end && *end == '\0') v = (size_t)x;
+    }
+    cached = 1;
+  }
+  return v;
+}
+
+static int fused_eligible(const mofft_schedule *s) {
+  if (s->ks != PASS_KERNELS || s->pass_count != 2) return 0;
+  const int k = s->passes[0].kernel;
+  if (s->passes[1].kernel != k || !mofft_hip_fused_available(k)) return 0;
+  const mofft_registry_entry *e = &s->ks->entries[k];
+  if (e->rows || s->passes[0].stride % (size_t)e->columns_per_block != 0 ||
+      s->passes[1].stride % (size_t)e->columns_per_block != 0)
+    return 0;
+  if (s->batch < FUSED_MIN_BATCH) return 0;
+  return sat_mul(s->length, complex_bytes(s->precision_bits)) <= fused_max_bytes();
+}
+
+static int fused_ready(mofft_plan_common *p) {
+  if (p->fused_state) return p->fused_state > 0;
+  p->fused_state = -1;
+  if (!fused_eligible(&p->s)) return 0;
+  int bpc = 0, cus = 0;
+  if (mofft_hip_fused_occupancy(p->s.passes[0].kernel, &bpc, &cus) != 0 ||
+      bpc <= 0 || cus <= 0)
+    return 0;
+  const size_t slot_bytes =
+      sat_mul(p->s.length, complex_bytes(p->s.precision_bits));
+  const size_t ws = sat_mul((size_t)FUSED_TEAMS * FUSED_SLOTS, slot_bytes);
+  const int64_t words =
+      mofft_fused_counter_words(FUSED_TEAMS, (uint32_t)(p->s.batch + 2));
+  if (mofft_hip_malloc(&p->fused_work, ws + (size_t)words * 4) != 0) {
+    p->fused_work = NULL;
+    return 0;
+  }
+  p->fused_slot_bytes = slot_bytes;
+  p->fused_grid = bpc * cus;
+  p->fused_state = 1;
+  return 1;
+}
+
+static int launch_fused(const mofft_plan_common *p, const void *in, void *out,
+                        void *stream) {
+  const mofft_schedule *s = &p->s;
+  const mofft_registry_entry *e = &s->ks->entries[s->passes[0].kernel];
+  mofft_fused_args fa;
+  memset(&fa, 0, sizeof(fa));
+  for (int i = 0; i < 2; ++i) fill_pass_args(p, i, NULL, NULL, 1, &fa.pass[i]);
+  fa.in = in;
+  fa.out = out;
+  fa.work = p->fused_work;
+  fa.counters = (unsigned int *)((char *)p->fused_work +
+                                 FUSED_TEAMS * FUSED_SLOTS * p->fused_slot_bytes);
+  fa.batch = (int64_t)s->batch;
+  fa.teams = FUSED_TEAMS;
+  fa.slots = FUSED_SLOTS;
+  fa.tiles = (uint32_t)(s->passes[0].stride / (size_t)e->columns_per_block);
+  fa.max_slots = (uint32_t)(s->batch + 2);
+  fa.spin_limit = (uint64_t)1 << 20;
+  return mofft_hip_fused_launch(s->passes[0].kernel, &fa, p->fused_grid, stream);
+}
+
+static int fused_enabled(void) {
+  static int cached = -1;
+  if (cached < 0) {
+    const char *env = getenv("MOFFT_FUSED");
+    cached = env && *env ? atoi(env) != 0 : 0;
+  }
+  return cached;
+}
+
+/* Chunk pipeline: consecutive chunks alternate between the caller's stream
+ * and the plan's second stream (each with its own workspace), so one
+ * chunk's passes overlap the next chunk's and no launch drains the device
+ * alone.  Chunks are disjoint transforms; the two streams fork from and
+ * join back to the caller's stream. */
+static int pipeline_enabled(void) {
+  static int cached = -1;
+  if (cached < 0) {
+    const char *env = getenv("MOFFT_PIPELINE");
+    cached = env && *env ? atoi(env) != 0 : 0;
+  }
+  return cached;
+}
+
+static int pipe_ready(mofft_plan_common *p) {
+  if (p->pipe_state) return p->pipe_state > 0;
+  p->pipe_state = -1;
+  const size_t ws = schedule_workspace(&p->s, p->placement);
+  if ((ws && mofft_hip_malloc(&p->pipe_workspace, ws) != 0) ||
+      mofft_hip_stream_create(&p->pipe_stream) != 0 ||
+      mofft_hip_event_create(&p->pipe_fork) != 0 ||
+      mofft_hip_event_create(&p->pipe_join) != 0) {
+    pipe_release(p);
+    p->pipe_state = -1;
+    return 0;
+  }
+  p->pipe_state = 1;
+  return 1;
+}
+
+static int launch_schedule_pipelined(const mofft_plan_common *cp,
+                                     const void *in, void *out, void *stream) {
+  mofft_plan_common *p = (mofft_plan_common *)cp;
+  const mofft_schedule *s = &p->s;
+  if (mofft_hip_event_record(p->pipe_fork, stream) != 0 ||
+      mofft_hip_stream_wait_event(p->pipe_stream, p->pipe_fork) != 0)
+    return 1;
+  size_t k = 0;
+  for (size_t c0 = 0; c0 < s->batch; c0 += s->chunk, ++k) {
+    const size_t cnt = s->batch - c0 < s->chunk ? s->batch - c0 : s->chunk;
+    const int odd = (int)(k & 1);
+    const int rc = launch_chunk(p, in, out, odd ? p->pipe_workspace : p->workspace,
+                                odd ? p->pipe_stream : stream, c0, cnt, NULL);
+    if (rc != 0) return rc;
+  }
+  if (mofft_hip_event_record(p->pipe_join, p->pipe_stream) != 0 ||
+      mofft_hip_stream_wait_event(stream, p->pipe_join) != 0)
+    return 1;
   return 0;
 }
 
@@ -1293,6 +1484,13 @@ static mofft_status execute_async(const mofft_plan_common *p, const void *in,
   if (own_workspace) {
     work = p->workspace;
     work_bytes = schedule_workspace(&p->s, p->placement);
+    if (fused_enabled() && fused_ready((mofft_plan_common *)p))
+      return launch_fused(p, in, out, stream) == 0 ? MOFFT_SUCCESS
+                                                   : MOFFT_INTERNAL_ERROR;
+    if (pipeline_enabled() && p->s.pass_count > 1 && p->s.batch > p->s.chunk &&
+        pipe_ready((mofft_plan_common *)p))
+      return launch_schedule_pipelined(p, in, out, stream) == 0
+                 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
   }
   return execute_device(p, in, out, work, work_bytes, stream);
 }
@@ -1476,6 +1674,8 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
   }
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
+  pipe_release(p);
+  fused_release(p);
   p->s = s;
   p->twiddles = tmp.twiddles;
   p->workspace = tmp.workspace;
