@@ -182,34 +182,12 @@ def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
     return "\n".join(lines)
 
 
-# Column kernels short enough for a two-pass transform of this length to
-# stay in one XCD's L2 also get the XCD-resident fused launcher (fused.h).
-FUSED_MAX_LENGTH = 512
-
-
-def fused_eligible(e: registry.Entry) -> bool:
-    return (not e.rows and e.variant != registry.MFMA_VARIANT
-            and e.length <= FUSED_MAX_LENGTH)
-
-
-def _fused_targs(e: registry.Entry) -> str:
-    t = "float" if e.precision == 32 else "double"
-    stages = ", ".join(str(r) for r in e.stages)
-    return f"{t}, {e.length}, {e.tpt}, {e.cols}, {stages}"
-
-
 def emit_kernel_part(es: list[registry.Entry], indices: list[int]) -> str:
-    lines = [HEADER, '#include "../hip/fused.h"', "", "namespace mofft {"]
+    lines = [HEADER, '#include "../hip/fft_pass.h"', "", "namespace mofft {"]
     for i in indices:
         e = es[i]
         lines.append(f"int launch_entry_{i}(const mofft_pass_args *a, void *s) {{"
                      f" return PassKernel<{_targs(e)}>::launch(a, (hipStream_t)s); }}")
-        if fused_eligible(e):
-            k = f"FusedKernel<{_fused_targs(e)}>"
-            lines.append(f"int fused_entry_{i}(const mofft_fused_args *a, int g, void *s) {{"
-                         f" return {k}::launch(a, g, (hipStream_t)s); }}")
-            lines.append(f"int fused_occupancy_{i}(int *b, int *c) {{"
-                         f" return {k}::occupancy(b, c); }}")
     lines += ["}  // namespace mofft", ""]
     return "\n".join(lines)
 
@@ -219,24 +197,12 @@ def emit_launch_table(es: list[registry.Entry]) -> str:
              "namespace mofft {"]
     for i in range(len(es)):
         lines.append(f"int launch_entry_{i}(const mofft_pass_args *, void *);")
-    for i, e in enumerate(es):
-        if fused_eligible(e):
-            lines.append(f"int fused_entry_{i}(const mofft_fused_args *, int, void *);")
-            lines.append(f"int fused_occupancy_{i}(int *, int *);")
     lines += ["", "typedef int (*launch_fn)(const mofft_pass_args *, void *);",
               "extern const launch_fn launch_table[] = {"]
     for i in range(len(es)):
         lines.append(f"  launch_entry_{i},")
-    lines += ["};", f"extern const int launch_table_size = {len(es)};", "",
-              "typedef int (*fused_fn)(const mofft_fused_args *, int, void *);",
-              "typedef int (*fused_occupancy_fn)(int *, int *);",
-              "extern const fused_fn fused_table[] = {"]
-    for i, e in enumerate(es):
-        lines.append(f"  fused_entry_{i}," if fused_eligible(e) else "  nullptr,")
-    lines += ["};", "extern const fused_occupancy_fn fused_occupancy_table[] = {"]
-    for i, e in enumerate(es):
-        lines.append(f"  fused_occupancy_{i}," if fused_eligible(e) else "  nullptr,")
-    lines += ["};", "}  // namespace mofft", ""]
+    lines += ["};", f"extern const int launch_table_size = {len(es)};",
+              "}  // namespace mofft", ""]
     return "\n".join(lines)
 
 

@@ -95,7 +95,6 @@ __device__ __forceinline__ void setup_gather(Ctx<T> &x,
   x.swap_out = (flags & MOFFT_PASS_SWAP_OUT) != 0;
   x.pass_tw = false;
   x.pass_table = false;
-  x.coherent = false;
 }
 
 /* One queue tile of kind <T, L, TPT, C, ROWS, R...>.  Row kinds are always

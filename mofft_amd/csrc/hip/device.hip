@@ -17,11 +17,6 @@ namespace mofft {
 typedef int (*launch_fn)(const mofft_pass_args *, void *);
 extern const launch_fn launch_table[];
 extern const int launch_table_size;
-/* XCD-resident two-pass launchers (nullptr where a kernel has none) */
-typedef int (*fused_fn)(const mofft_fused_args *, int, void *);
-typedef int (*fused_occupancy_fn)(int *, int *);
-extern const fused_fn fused_table[];
-extern const fused_occupancy_fn fused_occupancy_table[];
 /* persistent batch kernels (generated mofft_batch_f32/f64.hip) */
 int batch_occupancy_f32(int lds, int *bpc, int *cus);
 int batch_occupancy_f64(int lds, int *bpc, int *cus);
@@ -230,23 +225,6 @@ int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
   if (kernel < 0 || kernel >= mofft::launch_table_size)
     return (int)hipErrorInvalidValue;
   return mofft::launch_table[kernel](args, stream);
-}
-
-int mofft_hip_fused_available(int kernel) {
-  return kernel >= 0 && kernel < mofft::launch_table_size &&
-         mofft::fused_table[kernel] != nullptr;
-}
-
-int mofft_hip_fused_occupancy(int kernel, int *blocks_per_cu, int *cu_count) {
-  if (!mofft_hip_fused_available(kernel) || !blocks_per_cu || !cu_count)
-    return (int)hipErrorInvalidValue;
-  return mofft::fused_occupancy_table[kernel](blocks_per_cu, cu_count);
-}
-
-int mofft_hip_fused_launch(int kernel, const mofft_fused_args *args, int grid,
-                           void *stream) {
-  if (!mofft_hip_fused_available(kernel)) return (int)hipErrorInvalidValue;
-  return mofft::fused_table[kernel](args, grid, stream);
 }
 
 int mofft_hip_batch_occupancy(int precision_bits, int lds_bytes,

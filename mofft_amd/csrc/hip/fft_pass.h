@@ -115,8 +115,6 @@ struct Ctx {
   int c;
   int t;
   bool live;
-  bool coherent;      /* input written in this launch by another CU of the
-                         same XCD: read it from the L2, not the CU's L1 */
   bool swap_in;
   bool swap_out;
   bool pass_tw;
@@ -174,31 +172,6 @@ __device__ __forceinline__ void cmul(T &re, T &im, const cx<T> w) {
   const T i = re * w.y + im * w.x;
   re = r;
   im = i;
-}
-
-/* Element load that misses the CU's L1 and reads the L2 (relaxed
- * agent-scope atomic loads of the 8-byte words). */
-template <typename T>
-__device__ __forceinline__ cx<T> ld_l2(const cx<T> *p) {
-  const unsigned long long *w = reinterpret_cast<const unsigned long long *>(p);
-  cx<T> r;
-  if constexpr (sizeof(T) == 4) {
-    const unsigned long long v = __hip_atomic_load(
-        const_cast<unsigned long long *>(w), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-    r.x = __uint_as_float((unsigned)v);
-    r.y = __uint_as_float((unsigned)(v >> 32));
-  } else {
-    const unsigned long long a = __hip_atomic_load(
-        const_cast<unsigned long long *>(w), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long b = __hip_atomic_load(
-        const_cast<unsigned long long *>(w + 1), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT);
-    r.x = __longlong_as_double((long long)a);
-    r.y = __longlong_as_double((long long)b);
-  }
-  return r;
 }
 
 template <typename T>
@@ -336,7 +309,6 @@ struct Stage {
    * load (data, then pass twiddles) is issued before any result is used:
    * a use between two loads makes the compiler wait for the first before
    * issuing the second, one memory round trip per element. */
-  template <bool L2>
   static __device__ __forceinline__ void load_raw(const Ctx<T> &x,
                                                   cx<T> (&raw)[IT][R]) {
 #pragma unroll
@@ -350,7 +322,7 @@ struct Stage {
         v.y = T(0);
         if (ok) {
           const cx<T> *q = x.src + (uint32_t)(jj + k * NB) * x.stride;
-          v = L2 ? ld_l2<T>(q) : ld_stream<T>(q);
+          v = ld_stream<T>(q);
         }
         raw[it][k] = v;
       }
@@ -384,8 +356,7 @@ struct Stage {
       }
     } else {
       cx<T> raw[IT][R];
-      if (x.coherent) load_raw<true>(x, raw);
-      else load_raw<false>(x, raw);
+      load_raw(x, raw);
       /* w_N^(e * (j % ns) * tw_scale) per element (1 where unused) */
       cx<T> w[IT][R];
 #pragma unroll
@@ -589,7 +560,6 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.pass_tw = (p.flags & MOFFT_PASS_TWIDDLE) != 0 && jr != 0;
   x.pass_table = (p.flags & MOFFT_PASS_TWIDDLE_TABLE) != 0;
   x.tpass = (const cx<T> *)p.tw_pass + jr;
-  x.coherent = false;
 }
 
 /* Load, transform and store the tile described by x. */

@@ -109,12 +109,6 @@ typedef struct mofft_plan_common {
   void *pipe_fork;
   void *pipe_join;
   int pipe_state;
-  /* XCD-resident two-pass execution: team workspace slots followed by the
-   * counters; fused_state 0 = not set up, 1 = ready, -1 = not used */
-  void *fused_work;
-  size_t fused_slot_bytes;
-  int fused_grid;
-  int fused_state;
   void *staging_in;
   void *staging_out;
   const void *bound_input;
@@ -555,16 +549,9 @@ static void pipe_release(mofft_plan_common *p) {
   p->pipe_state = 0;
 }
 
-static void fused_release(mofft_plan_common *p) {
-  mofft_hip_free(p->fused_work);
-  p->fused_work = NULL;
-  p->fused_state = 0;
-}
-
 static void plan_release(mofft_plan_common *p) {
   if (!p) return;
   pipe_release(p);
-  fused_release(p);
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
   mofft_hip_free(p->staging_in);
@@ -752,95 +739,6 @@ static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
     if (rc != 0) return rc;
   }
   return 0;
-}
-
-/* XCD-resident two-pass execution (mofft_abi.h, hip/fused.h): teams of
- * workgroups, one per XCD, each with FUSED_SLOTS workspace slots. */
-#define FUSED_TEAMS 8u
-#define FUSED_SLOTS 3u
-/* Largest transform (bytes) run fused: the slots a team has in flight stay
- * within its XCD's 4 MB L2.  MOFFT_FUSED_BYTES overrides it (0 = never). */
-#define FUSED_MAX_BYTES ((size_t)1 << 20)
-#define FUSED_MIN_BATCH 16u
-
-static size_t fused_max_bytes(void) {
-  static int cached = 0;
-  static size_t v = FUSED_MAX_BYTES;
-  if (!cached) {
-    const char *env = getenv("MOFFT_FUSED_BYTES");
-    if (env && *env) {
-      char *end = NULL;
-      const unsigned long long x = strtoull(env, &end, 10);
-      if (end && *end == '\0') v = (size_t)x;
-    }
-    cached = 1;
-  }
-  return v;
-}
-
-static int fused_eligible(const mofft_schedule *s) {
-  if (s->ks != PASS_KERNELS || s->pass_count != 2) return 0;
-  const int k = s->passes[0].kernel;
-  if (s->passes[1].kernel != k || !mofft_hip_fused_available(k)) return 0;
-  const mofft_registry_entry *e = &s->ks->entries[k];
-  if (e->rows || s->passes[0].stride % (size_t)e->columns_per_block != 0 ||
-      s->passes[1].stride % (size_t)e->columns_per_block != 0)
-    return 0;
-  if (s->batch < FUSED_MIN_BATCH) return 0;
-  return sat_mul(s->length, complex_bytes(s->precision_bits)) <= fused_max_bytes();
-}
-
-static int fused_ready(mofft_plan_common *p) {
-  if (p->fused_state) return p->fused_state > 0;
-  p->fused_state = -1;
-  if (!fused_eligible(&p->s)) return 0;
-  int bpc = 0, cus = 0;
-  if (mofft_hip_fused_occupancy(p->s.passes[0].kernel, &bpc, &cus) != 0 ||
-      bpc <= 0 || cus <= 0)
-    return 0;
-  const size_t slot_bytes =
-      sat_mul(p->s.length, complex_bytes(p->s.precision_bits));
-  const size_t ws = sat_mul((size_t)FUSED_TEAMS * FUSED_SLOTS, slot_bytes);
-  const int64_t words =
-      mofft_fused_counter_words(FUSED_TEAMS, (uint32_t)(p->s.batch + 2));
-  if (mofft_hip_malloc(&p->fused_work, ws + (size_t)words * 4) != 0) {
-    p->fused_work = NULL;
-    return 0;
-  }
-  p->fused_slot_bytes = slot_bytes;
-  p->fused_grid = bpc * cus;
-  p->fused_state = 1;
-  return 1;
-}
-
-static int launch_fused(const mofft_plan_common *p, const void *in, void *out,
-                        void *stream) {
-  const mofft_schedule *s = &p->s;
-  const mofft_registry_entry *e = &s->ks->entries[s->passes[0].kernel];
-  mofft_fused_args fa;
-  memset(&fa, 0, sizeof(fa));
-  for (int i = 0; i < 2; ++i) fill_pass_args(p, i, NULL, NULL, 1, &fa.pass[i]);
-  fa.in = in;
-  fa.out = out;
-  fa.work = p->fused_work;
-  fa.counters = (unsigned int *)((char *)p->fused_work +
-                                 FUSED_TEAMS * FUSED_SLOTS * p->fused_slot_bytes);
-  fa.batch = (int64_t)s->batch;
-  fa.teams = FUSED_TEAMS;
-  fa.slots = FUSED_SLOTS;
-  fa.tiles = (uint32_t)(s->passes[0].stride / (size_t)e->columns_per_block);
-  fa.max_slots = (uint32_t)(s->batch + 2);
-  fa.spin_limit = (uint64_t)1 << 20;
-  return mofft_hip_fused_launch(s->passes[0].kernel, &fa, p->fused_grid, stream);
-}
-
-static int fused_enabled(void) {
-  static int cached = -1;
-  if (cached < 0) {
-    const char *env = getenv("MOFFT_FUSED");
-    cached = env && *env ? atoi(env) != 0 : 0;
-  }
-  return cached;
 }
 
 /* Chunk pipeline: consecutive chunks alternate between the caller's stream
@@ -1484,9 +1382,6 @@ static mofft_status execute_async(const mofft_plan_common *p, const void *in,
   if (own_workspace) {
     work = p->workspace;
     work_bytes = schedule_workspace(&p->s, p->placement);
-    if (fused_enabled() && fused_ready((mofft_plan_common *)p))
-      return launch_fused(p, in, out, stream) == 0 ? MOFFT_SUCCESS
-                                                   : MOFFT_INTERNAL_ERROR;
     if (pipeline_enabled() && p->s.pass_count > 1 && p->s.batch > p->s.chunk &&
         pipe_ready((mofft_plan_common *)p))
       return launch_schedule_pipelined(p, in, out, stream) == 0
@@ -1675,7 +1570,6 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
   pipe_release(p);
-  fused_release(p);
   p->s = s;
   p->twiddles = tmp.twiddles;
   p->workspace = tmp.workspace;

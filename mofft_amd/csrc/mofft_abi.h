@@ -127,54 +127,6 @@ int mofft_hip_fill_pass(void *out, int precision_bits, int64_t ns, int64_t L,
  * LDS), as the first launch would; nothing is enqueued. */
 int mofft_hip_launch_pass(int kernel, const mofft_pass_args *args,
                           void *stream);
-/* ---- XCD-resident two-pass execution ----
- *
- * A two-pass schedule whose passes use the same column kernel can run as
- * ONE persistent launch in which the intermediate array never leaves the
- * L2 of the XCD that produced it.  Every workgroup reads the XCD it runs on
- * (hardware register XCC_ID) and joins that XCD's team.  A team takes whole
- * transforms from a launch-wide counter; each transform passes through one
- * of the team's `slots` workspace slots (pass 0: input -> slot, pass 1:
- * slot -> output).  A team's queue interleaves pass-0 tiles of its slot r
- * with pass-1 tiles of slot r - 1, so a pass-1 tile reads what its XCD
- * wrote one round earlier, while it is still in that XCD's L2.  Within a
- * team a producer waits for its stores to complete, then increments the
- * slot's counter with an atomic; a consumer polls the counter (atomics
- * only) and reads the slot with loads that miss the CU's L1 and hit the
- * shared L2. */
-typedef struct {
-  mofft_pass_args pass[2]; /* per-pass arguments of ONE transform; in and
-                              out are filled per tile by the kernel */
-  const void *in;          /* batch input, transform t at t * n elements */
-  void *out;               /* batch output */
-  void *work;              /* teams * slots transforms */
-  unsigned int *counters;  /* mofft_fused_counter_words() words */
-  int64_t batch;
-  uint32_t teams;          /* XCD id -> team: xcc % teams */
-  uint32_t slots;          /* workspace slots per team (>= 2) */
-  uint32_t tiles;          /* tiles per pass of one transform */
-  uint32_t max_slots;      /* slot-table entries per team */
-  uint64_t spin_limit;     /* polls before a wait gives up (error flag) */
-} mofft_fused_args;
-
-/* Counter words: [0] next transform, [1] error flag, [2, 2 + teams) team
- * queue heads, then per team max_slots published transforms (t + 1, 0 =
- * not yet) and max_slots x 2 finished-tile counters. */
-static inline int64_t mofft_fused_counter_words(uint32_t teams,
-                                                uint32_t max_slots) {
-  return 2 + (int64_t)teams + (int64_t)teams * max_slots * 3;
-}
-
-/* Whether registry entry `kernel` has a fused launcher. */
-int mofft_hip_fused_available(int kernel);
-/* Resident workgroups per CU of the fused kernel of `kernel`, and the CU
- * count of the device. */
-int mofft_hip_fused_occupancy(int kernel, int *blocks_per_cu, int *cu_count);
-/* Clear the counters (a kernel, no memset node) and run the fused kernel
- * of `kernel` with `grid` workgroups; args == NULL configures only. */
-int mofft_hip_fused_launch(int kernel, const mofft_fused_args *args, int grid,
-                           void *stream);
-
 /* Device-side interval timer on `stream` (HIP events): begin records the
  * start, end records the stop, waits for it and returns seconds. */
 int mofft_hip_timer_begin(void *stream);
