@@ -360,10 +360,6 @@ int mofft_hip_event_record(void *event, void *stream) {
   return (int)hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
 }
 
-int mofft_hip_stream_wait_event(void *stream, void *event) {
-  return (int)hipStreamWaitEvent((hipStream_t)stream, (hipEvent_t)event, 0);
-}
-
 int mofft_hip_event_query(void *event) {
   const hipError_t e = hipEventQuery((hipEvent_t)event);
   if (e == hipSuccess) return 1;

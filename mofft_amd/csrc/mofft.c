@@ -101,14 +101,6 @@ typedef struct mofft_plan_common {
   unsigned int kernel_bank;
   void *twiddles;
   void *workspace;
-  /* second stream of the chunk pipeline (odd chunks), its workspace and
-   * the fork/join events; pipe_state 0 = not set up, 1 = ready, -1 =
-   * unavailable */
-  void *pipe_stream;
-  void *pipe_workspace;
-  void *pipe_fork;
-  void *pipe_join;
-  int pipe_state;
   void *staging_in;
   void *staging_out;
   const void *bound_input;
@@ -540,18 +532,8 @@ const char *mofft_wisdom_source(void) { return mofft_generated_wisdom_source(); 
 /* plan lifetime                                                        */
 /* ------------------------------------------------------------------ */
 
-static void pipe_release(mofft_plan_common *p) {
-  if (p->pipe_stream) mofft_hip_stream_destroy(p->pipe_stream);
-  if (p->pipe_fork) mofft_hip_event_destroy(p->pipe_fork);
-  if (p->pipe_join) mofft_hip_event_destroy(p->pipe_join);
-  mofft_hip_free(p->pipe_workspace);
-  p->pipe_stream = p->pipe_fork = p->pipe_join = p->pipe_workspace = NULL;
-  p->pipe_state = 0;
-}
-
 static void plan_release(mofft_plan_common *p) {
   if (!p) return;
-  pipe_release(p);
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
   mofft_hip_free(p->staging_in);
@@ -738,57 +720,6 @@ static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
     const int rc = launch_chunk(p, in, out, work, stream, c0, cnt, timing);
     if (rc != 0) return rc;
   }
-  return 0;
-}
-
-/* Chunk pipeline: consecutive chunks alternate between the caller's stream
- * and the plan's second stream (each with its own workspace), so one
- * chunk's passes overlap the next chunk's and no launch drains the device
- * alone.  Chunks are disjoint transforms; the two streams fork from and
- * join back to the caller's stream. */
-static int pipeline_enabled(void) {
-  static int cached = -1;
-  if (cached < 0) {
-    const char *env = getenv("MOFFT_PIPELINE");
-    cached = env && *env ? atoi(env) != 0 : 0;
-  }
-  return cached;
-}
-
-static int pipe_ready(mofft_plan_common *p) {
-  if (p->pipe_state) return p->pipe_state > 0;
-  p->pipe_state = -1;
-  const size_t ws = schedule_workspace(&p->s, p->placement);
-  if ((ws && mofft_hip_malloc(&p->pipe_workspace, ws) != 0) ||
-      mofft_hip_stream_create(&p->pipe_stream) != 0 ||
-      mofft_hip_event_create(&p->pipe_fork) != 0 ||
-      mofft_hip_event_create(&p->pipe_join) != 0) {
-    pipe_release(p);
-    p->pipe_state = -1;
-    return 0;
-  }
-  p->pipe_state = 1;
-  return 1;
-}
-
-static int launch_schedule_pipelined(const mofft_plan_common *cp,
-                                     const void *in, void *out, void *stream) {
-  mofft_plan_common *p = (mofft_plan_common *)cp;
-  const mofft_schedule *s = &p->s;
-  if (mofft_hip_event_record(p->pipe_fork, stream) != 0 ||
-      mofft_hip_stream_wait_event(p->pipe_stream, p->pipe_fork) != 0)
-    return 1;
-  size_t k = 0;
-  for (size_t c0 = 0; c0 < s->batch; c0 += s->chunk, ++k) {
-    const size_t cnt = s->batch - c0 < s->chunk ? s->batch - c0 : s->chunk;
-    const int odd = (int)(k & 1);
-    const int rc = launch_chunk(p, in, out, odd ? p->pipe_workspace : p->workspace,
-                                odd ? p->pipe_stream : stream, c0, cnt, NULL);
-    if (rc != 0) return rc;
-  }
-  if (mofft_hip_event_record(p->pipe_join, p->pipe_stream) != 0 ||
-      mofft_hip_stream_wait_event(stream, p->pipe_join) != 0)
-    return 1;
   return 0;
 }
 
@@ -1382,10 +1313,6 @@ static mofft_status execute_async(const mofft_plan_common *p, const void *in,
   if (own_workspace) {
     work = p->workspace;
     work_bytes = schedule_workspace(&p->s, p->placement);
-    if (pipeline_enabled() && p->s.pass_count > 1 && p->s.batch > p->s.chunk &&
-        pipe_ready((mofft_plan_common *)p))
-      return launch_schedule_pipelined(p, in, out, stream) == 0
-                 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
   }
   return execute_device(p, in, out, work, work_bytes, stream);
 }
@@ -1569,7 +1496,6 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
   }
   mofft_hip_free(p->twiddles);
   mofft_hip_free(p->workspace);
-  pipe_release(p);
   p->s = s;
   p->twiddles = tmp.twiddles;
   p->workspace = tmp.workspace;

@@ -212,8 +212,6 @@ int mofft_hip_stream_destroy(void *stream);
 int mofft_hip_event_create(void **event);
 int mofft_hip_event_destroy(void *event);
 int mofft_hip_event_record(void *event, void *stream);
-/* Work submitted to `stream` after this call waits for `event`. */
-int mofft_hip_stream_wait_event(void *stream, void *event);
 /* 1 when the event has completed, 0 when not yet, negative on error. */
 int mofft_hip_event_query(void *event);
 /* An event that records timestamps, and the seconds between two of them
