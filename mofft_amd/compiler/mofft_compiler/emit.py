@@ -161,6 +161,7 @@ def emit_registry_c(es: list[registry.Entry], bs: list[registry.Entry]) -> str:
 def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
     """One precision's persistent batch kernel: a switch over its kinds."""
     t = "float" if precision == 32 else "double"
+    asserts: list[str] = []
     lines = [HEADER, '#include "../hip/batch.h"', "", "namespace mofft {",
              f"struct BatchDispatchF{precision} {{",
              f"  static __device__ __forceinline__ void run(int kind, const mofft_batch_segment &sg,",
@@ -170,7 +171,9 @@ def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
         if e.precision != precision:
             continue
         lines.append(f"      case {i}: batch_tile<{_targs(e)}>(sg, block, lds); return;")
-    lines += ["      default: return;", "    }", "  }", "};", "",
+        asserts.append(f"static_assert(StageTw<{_targs(e)}>::BYTES == {e.lds_bytes},"
+                       f' "registry LDS bytes of batch kind {i}");')
+    lines += ["      default: return;", "    }", "  }", "};", ""] + asserts + ["",
               f"__global__ void __launch_bounds__(kBatchThreads) MOFFT_BATCH_WAVES_F{precision}",
               f"batch_kernel_f{precision}(mofft_batch_header h) {{",
               f"  batch_loop<{t}, BatchDispatchF{precision}>(h);", "}", "",
@@ -188,6 +191,8 @@ def emit_kernel_part(es: list[registry.Entry], indices: list[int]) -> str:
         e = es[i]
         lines.append(f"int launch_entry_{i}(const mofft_pass_args *a, void *s) {{"
                      f" return PassKernel<{_targs(e)}>::launch(a, (hipStream_t)s); }}")
+        lines.append(f"static_assert(PassKernel<{_targs(e)}>::LDS == {e.lds_bytes},"
+                     f' "registry LDS bytes of entry {i}");')
     lines += ["}  // namespace mofft", ""]
     return "\n".join(lines)
 

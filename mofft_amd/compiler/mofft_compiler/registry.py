@@ -60,13 +60,31 @@ class Entry:
         return 8 if self.precision == 32 else 16
 
     @property
+    def stage_tw_entries(self) -> int:
+        """Entries of the sub-stage twiddle table (sub-stages with ns > 1)."""
+        ns, count = 1, 0
+        for r in self.stages:
+            if ns > 1:
+                count += ns * (r - 1)
+            ns *= r
+        return count
+
+    @property
     def lds_bytes(self) -> int:
+        """Tile exchange area, plus the sub-stage twiddle table when it is at
+        most a quarter of that and the sum stays within LDS_CAP (the rule of
+        StageTw in csrc/hip/fft_pass.h)."""
         real = self.complex_bytes // 2
         lp = self.length + (self.length >> 4) + 1
         row_words = self.cols * lp
         col_words = self.length * self.cols + ((self.length * self.cols) >> 5) + 1
         words = row_words if self.rows else max(row_words, col_words)
-        return words * real
+        tile = words * real
+        table = self.stage_tw_entries * self.complex_bytes
+        offset = (tile + 15) // 16 * 16
+        if table > 0 and 4 * table <= tile and offset + table <= LDS_CAP:
+            return offset + table
+        return tile
 
     @property
     def threads(self) -> int:

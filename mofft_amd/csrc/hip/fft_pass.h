@@ -562,15 +562,70 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.tpass = (const cx<T> *)p.tw_pass + jr;
 }
 
+/* Sub-stage twiddle table of a kernel: entries OFF + (jj % NS) * (R - 1) + k
+ * for every sub-stage with NS > 1, in execution order.  When it is small
+ * next to the tile (at most a quarter of its LDS, and within the LDS cap)
+ * each tile copies it into LDS behind the tile area, loading it together
+ * with its data, so the sub-stages read their twiddles at LDS latency
+ * instead of waiting for a global load after every exchange.  The registry
+ * (mofft_compiler/registry.py, Entry.lds_bytes) sizes LDS by the same rule. */
+constexpr int kLdsCap = 150 * 1024;
+
+template <int... R>
+constexpr int stage_tw_entries() {
+  constexpr int rs[] = {radix_value(R)...};
+  int ns = 1, count = 0;
+  for (int i = 0; i < (int)sizeof...(R); ++i) {
+    if (ns > 1) count += ns * (rs[i] - 1);
+    ns *= rs[i];
+  }
+  return count;
+}
+
+template <typename T, int L, int TPT, int C, bool ROWS, int... R>
+struct StageTw {
+  static constexpr int COUNT = stage_tw_entries<R...>();
+  static constexpr int TILE = Tile<T, L, TPT, C, ROWS>::BYTES;
+  static constexpr int OFFSET = (TILE + 15) / 16 * 16;
+  static constexpr int TABLE = COUNT * (int)sizeof(cx<T>);
+  static constexpr bool IN_LDS =
+      COUNT > 0 && 4 * TABLE <= TILE && OFFSET + TABLE <= kLdsCap;
+  static constexpr int BYTES = IN_LDS ? OFFSET + TABLE : TILE;
+};
+
 /* Load, transform and store the tile described by x. */
 template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
 __device__ __forceinline__ void run_tile(const Ctx<T> &x,
                                          const mofft_pass_args &p) {
   using First = typename StageSel<T, L, TPT, C, ROWS, 1, 0, R0, R...>::type;
+  using SW = StageTw<T, L, TPT, C, ROWS, R0, R...>;
   T vr[First::IT][First::RAD];
   T vi[First::IT][First::RAD];
-  First::load_first(x, vr, vi);
-  First::run(x, p, vr, vi);
+  if constexpr (SW::IN_LDS) {
+    constexpr int NT = TPT * C;
+    constexpr int PER = (SW::COUNT + NT - 1) / NT;
+    cx<T> tw[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = x.tid + i * NT;
+      if (q < SW::COUNT) tw[i] = x.stw[q];
+    }
+    First::load_first(x, vr, vi);
+    cx<T> *table = reinterpret_cast<cx<T> *>(
+        reinterpret_cast<unsigned char *>(x.lds) + SW::OFFSET);
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int q = x.tid + i * NT;
+      if (q < SW::COUNT) table[q] = tw[i];
+    }
+    /* read only after the first exchange's barriers */
+    Ctx<T> y = x;
+    y.stw = table;
+    First::run(y, p, vr, vi);
+  } else {
+    First::load_first(x, vr, vi);
+    First::run(x, p, vr, vi);
+  }
 }
 
 template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
@@ -589,7 +644,7 @@ struct PassKernel {
   static constexpr int THREADS = TPT * C;
   static constexpr int STAGES = sizeof...(R);
   static constexpr bool HAS_MFMA = ((R == kMfma16) || ...);
-  static constexpr int LDS = Tile<T, L, TPT, C, ROWS>::BYTES;
+  static constexpr int LDS = StageTw<T, L, TPT, C, ROWS, R...>::BYTES;
   static int launch(const mofft_pass_args *a, hipStream_t s) {
     static bool configured = false;
     if (!configured) {
