@@ -400,3 +400,16 @@ def test_invariant_divisor_multipliers(tmp_path):
     res = subprocess.run([str(exe)], capture_output=True, text=True)
     assert res.returncode == 0, res.stdout
     assert res.stdout.strip() == "bad 0"
+
+
+def test_launch_element_bound():
+    """Element offsets inside one launch are 32-bit: lengths above 2^30 are
+    rejected, and a batch is split so that one launch covers fewer than 2^30
+    elements (single-pass plans included)."""
+    with pytest.raises(mf.MofftError) as e:
+        mf.describe((1 << 30) * 2, 1, 32)
+    assert e.value.status == mf.UNSUPPORTED_SIZE
+    d = mf.describe(64, 1 << 25, 32)              # 2^31 elements in all
+    assert d.pass_count == 1 and d.chunk_transforms == (1 << 30) // 64
+    d = mf.describe(1 << 12, 1 << 20, 64)
+    assert d.chunk_transforms * (1 << 12) <= 1 << 30
