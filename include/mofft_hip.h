@@ -252,7 +252,11 @@ void mofft_batch_plan_destroy(mofft_batch_plan *plan);
  * mixed length, batch and precision against a device-memory budget and
  * runs them with continuous batching: requests that arrive while others
  * are on the GPU are admitted (FIFO) as soon as an in-flight slot and
- * budget free up, and run concurrently on several streams.  Plans (and
+ * budget free up, and run concurrently on several streams.  When a request
+ * is admitted, later pending requests of the same length, precision and
+ * direction whose input the server generates join it, up to `max_batch`
+ * transforms: one arena block and one batched execution serve them all,
+ * each still completed and released on its own.  Plans (and
  * their twiddles) stay resident per (length, precision, direction);
  * request buffers come from one device arena of `budget_bytes` allocated
  * at creation, so serving calls no device allocator.  Results are
@@ -264,9 +268,13 @@ typedef struct {
   size_t budget_bytes;    /* device arena for request buffers */
   uint32_t streams;       /* device streams; 0 = 4 */
   uint32_t max_inflight;  /* requests on the device at once; 0 = 4 * streams */
-  uint32_t max_batch;     /* largest batch of one request; 0 = 64 */
-  uint32_t flags;         /* reserved, 0 */
+  uint32_t max_batch;     /* largest batch of one request or execution;
+                             0 = 64 */
+  uint32_t flags;         /* MOFFT_SERVER_* */
 } mofft_server_options;
+
+/* Run every request as its own execution (no coalescing). */
+#define MOFFT_SERVER_NO_COALESCE 1u
 
 typedef struct {
   size_t length;
@@ -304,6 +312,8 @@ typedef struct {
   size_t peak_inflight;
   double flops_completed;   /* 5 N log2 N per completed transform */
   double bytes_completed;   /* 2 N sizeof(complex) per completed transform */
+  size_t executions;        /* batched executions enqueued (one or more
+                               requests each) */
 } mofft_server_stats;
 
 void mofft_server_options_init(mofft_server_options *options);

@@ -164,7 +164,10 @@ class ServerStats(ctypes.Structure):
                 ("resident_plans", ctypes.c_size_t), ("bytes_in_use", ctypes.c_size_t),
                 ("peak_bytes_in_use", ctypes.c_size_t), ("budget_bytes", ctypes.c_size_t),
                 ("peak_inflight", ctypes.c_size_t), ("flops_completed", ctypes.c_double),
-                ("bytes_completed", ctypes.c_double)]
+                ("bytes_completed", ctypes.c_double), ("executions", ctypes.c_size_t)]
+
+
+SERVER_NO_COALESCE = 1
 
 
 BATCH_NO_GRAPH = 1
@@ -570,13 +573,14 @@ class Server:
     a device-memory budget, resident plans, results in server buffers."""
 
     def __init__(self, budget_bytes: int, streams: int = 0, max_inflight: int = 0,
-                 max_batch: int = 0):
+                 max_batch: int = 0, coalesce: bool = True):
         o = ServerOptions()
         lib().mofft_server_options_init(ctypes.byref(o))
         o.budget_bytes = int(budget_bytes)
         o.streams = streams
         o.max_inflight = max_inflight
         o.max_batch = max_batch
+        o.flags = 0 if coalesce else SERVER_NO_COALESCE
         self._h = ctypes.c_void_p()
         _check(lib().mofft_server_create(ctypes.byref(self._h), ctypes.byref(o)),
                "server_create")

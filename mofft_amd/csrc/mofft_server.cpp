@@ -4,19 +4,25 @@
  * Continuous batching of 1-D C2C requests on one GPU:
  *
  *   submit   -> pending FIFO (never blocks)
- *   admit    -> while the head request fits (an in-flight slot is free and
- *               the arena has room for its input and output), carve its
- *               buffers from the arena, pick the least-loaded stream, enqueue
- *               [generate input] -> plan execution -> event
- *   retire   -> events that completed become completions; buffers return
- *               to the arena on release, which lets the next head in
+ *   admit    -> while the head request fits (in-flight slots free and the
+ *               arena has room for its input, output and workspace), carve
+ *               one block from the arena for it and for the later pending
+ *               requests of the same plan (length, precision, direction)
+ *               with generated input that fit in `max_batch` transforms,
+ *               pick the least-loaded stream and enqueue [generate each
+ *               input] -> one batched plan execution -> event
+ *   retire   -> an execution whose event completed yields one completion
+ *               per request; a block returns to the arena when the last of
+ *               its requests is released, which lets the next head in
  *
  * Requests therefore join while others are running and overlap with them
- * on the device (several streams), and admission is bounded by the budget
- * - the arena is the only device memory request data ever uses, so the
- * serving path calls no device allocator and never synchronises the device.
- * Plans are resident per (length, precision, direction), built for
- * `max_batch` transforms and executed for each request's batch.
+ * on the device (several streams), small requests of one plan share
+ * launches, and admission is bounded by the budget - the arena is the only
+ * device memory request data ever uses, so the serving path calls no
+ * device allocator and never synchronises the device.  Plans are resident
+ * per (length, precision, direction), built for `max_batch` transforms and
+ * executed for each group's total batch; a transform's result does not
+ * depend on the transforms executed beside it.
  *
  * Reference counterpart: none (the reference has no serving layer).
  */
@@ -25,6 +31,7 @@
 #include <string.h>
 #include <time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <deque>
 #include <map>
@@ -101,11 +108,25 @@ struct Job {
   mofft_request req;
   double submit;
   double start = 0;
+  size_t out_off = 0;     /* this request's output inside its block */
+  size_t out_bytes = 0;
+  uint64_t block = 0;     /* arena block shared with its execution group */
+  void *plan = nullptr;
+};
+
+/* Arena ranges of one execution: input, output and workspace of all its
+ * requests; freed when the last of them is released. */
+struct Block {
   size_t in_off = 0, out_off = 0, work_off = 0;
   size_t in_bytes = 0, out_bytes = 0, work_bytes = 0;
+  size_t refs = 0;
+};
+
+/* Requests executed together: one stream, one event. */
+struct Group {
+  std::vector<Job> jobs;
   int stream = -1;
   void *event = nullptr;
-  void *plan = nullptr;
 };
 
 struct PlanKey {
@@ -132,11 +153,14 @@ struct mofft_server {
   std::vector<void *> free_events;
   std::map<PlanKey, void *> plans;
   std::deque<Job> pending;
-  std::deque<Job> inflight;
+  std::deque<Group> inflight;
+  size_t inflight_jobs = 0;
   std::map<uint64_t, Job> done;     /* completed, buffers held until release */
+  std::map<uint64_t, Block> blocks;
   std::deque<mofft_completion> ready;
-  uint64_t next_ticket = 1;
+  uint64_t next_ticket = 1, next_block = 1;
   size_t submitted = 0, completed = 0, peak_bytes = 0, peak_inflight = 0;
+  size_t executions = 0;
   double flops = 0, bytes = 0;
 };
 
@@ -171,14 +195,15 @@ mofft_status get_plan(mofft_server *s, const mofft_request &r, void **plan) {
   return MOFFT_SUCCESS;
 }
 
-/* Arena bytes of a request: generated input, output, and the schedule's
- * workspace (three or more passes), each 256-byte aligned. */
-size_t request_bytes(const mofft_request &r, const void *plan, size_t *in_bytes,
-                     size_t *out_bytes, size_t *work_bytes) {
-  const size_t b = r.length * r.batch * complex_bytes(r.precision_bits);
+/* Arena bytes of `batch` transforms of request r's plan: generated input,
+ * output, and the schedule's workspace (three or more passes), each
+ * 256-byte aligned. */
+size_t group_bytes(const mofft_request &r, size_t batch, const void *plan,
+                   size_t *in_bytes, size_t *out_bytes, size_t *work_bytes) {
+  const size_t b = r.length * batch * complex_bytes(r.precision_bits);
   *out_bytes = b;
   *in_bytes = r.input ? 0 : b;
-  *work_bytes = mofft_internal_workspace_count(plan, r.batch);
+  *work_bytes = mofft_internal_workspace_count(plan, batch);
   return Arena::round(*in_bytes) + Arena::round(*out_bytes) +
          Arena::round(*work_bytes);
 }
@@ -198,96 +223,158 @@ mofft_completion completion_of(const mofft_server *s, const Job &j, double t,
   return c;
 }
 
+/* Drop one request's reference to its block; the last one frees it. */
 void release_job(mofft_server *s, const Job &j) {
-  if (j.in_bytes) s->arena.release(j.in_off, j.in_bytes);
-  if (j.out_bytes) s->arena.release(j.out_off, j.out_bytes);
-  if (j.work_bytes) s->arena.release(j.work_off, j.work_bytes);
+  auto it = s->blocks.find(j.block);
+  if (it == s->blocks.end()) return;
+  Block &b = it->second;
+  if (--b.refs > 0) return;
+  if (b.in_bytes) s->arena.release(b.in_off, b.in_bytes);
+  if (b.out_bytes) s->arena.release(b.out_off, b.out_bytes);
+  if (b.work_bytes) s->arena.release(b.work_off, b.work_bytes);
+  s->blocks.erase(it);
 }
 
-/* Move completed in-flight requests to the completion queue. */
+void finish_job(mofft_server *s, const Job &j, double t, int status) {
+  s->completed++;
+  if (status == MOFFT_SUCCESS) {
+    const double n = (double)j.req.length;
+    s->flops += 5.0 * n * std::log2(n) * (double)j.req.batch;
+    s->bytes += 2.0 * n * (double)complex_bytes(j.req.precision_bits) *
+                (double)j.req.batch;
+  }
+  s->ready.push_back(completion_of(s, j, t, status));
+  s->done[j.ticket] = j;
+}
+
+/* Move completed executions' requests to the completion queue. */
 void retire(mofft_server *s) {
   const double t = now_seconds();
   for (auto it = s->inflight.begin(); it != s->inflight.end();) {
     const int q = mofft_hip_event_query(it->event);
     if (q == 0) { ++it; continue; }
     const int status = q > 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
-    Job j = *it;
+    Group g = std::move(*it);
     it = s->inflight.erase(it);
-    s->free_events.push_back(j.event);
-    j.event = nullptr;
-    s->stream_load[j.stream]--;
-    s->completed++;
-    if (status == MOFFT_SUCCESS) {
-      const double n = (double)j.req.length;
-      s->flops += 5.0 * n * std::log2(n) * (double)j.req.batch;
-      s->bytes += 2.0 * n * (double)complex_bytes(j.req.precision_bits) *
-                  (double)j.req.batch;
-    }
-    s->ready.push_back(completion_of(s, j, t, status));
-    s->done[j.ticket] = j;
+    s->free_events.push_back(g.event);
+    s->stream_load[g.stream]--;
+    s->inflight_jobs -= g.jobs.size();
+    for (const Job &j : g.jobs) finish_job(s, j, t, status);
   }
 }
 
-/* Admit pending requests in FIFO order while slots and budget allow. */
+/* Carve the block of a group of `batch` transforms; false if it does not
+ * fit the arena now. */
+bool alloc_block(mofft_server *s, const mofft_request &r, size_t batch,
+                 const void *plan, Block *b) {
+  group_bytes(r, batch, plan, &b->in_bytes, &b->out_bytes, &b->work_bytes);
+  if (b->in_bytes && !s->arena.alloc(b->in_bytes, &b->in_off)) return false;
+  if (!s->arena.alloc(b->out_bytes, &b->out_off)) {
+    if (b->in_bytes) s->arena.release(b->in_off, b->in_bytes);
+    return false;
+  }
+  if (b->work_bytes && !s->arena.alloc(b->work_bytes, &b->work_off)) {
+    if (b->in_bytes) s->arena.release(b->in_off, b->in_bytes);
+    s->arena.release(b->out_off, b->out_bytes);
+    return false;
+  }
+  return true;
+}
+
+/* How far past the head admission looks for requests to join it. */
+constexpr size_t kCoalesceLookahead = 256;
+
+/* Admit pending requests in FIFO order while slots and budget allow; each
+ * admitted head takes same-plan pending requests with generated input
+ * along into its execution. */
 void admit(mofft_server *s) {
-  while (!s->pending.empty() && s->inflight.size() < s->opt.max_inflight) {
-    Job &j = s->pending.front();
-    request_bytes(j.req, j.plan, &j.in_bytes, &j.out_bytes, &j.work_bytes);
-    size_t in_off = 0, out_off = 0, work_off = 0;
-    if (j.in_bytes && !s->arena.alloc(j.in_bytes, &in_off)) return;
-    if (!s->arena.alloc(j.out_bytes, &out_off)) {
-      if (j.in_bytes) s->arena.release(in_off, j.in_bytes);
-      return;
+  const bool coalesce = (s->opt.flags & MOFFT_SERVER_NO_COALESCE) == 0;
+  while (!s->pending.empty() && s->inflight_jobs < s->opt.max_inflight) {
+    const Job &h = s->pending.front();
+    std::vector<size_t> members{0};
+    size_t total = h.req.batch;
+    if (coalesce && !h.req.input) {
+      const size_t end = std::min(s->pending.size(), kCoalesceLookahead);
+      for (size_t k = 1; k < end; ++k) {
+        if (s->inflight_jobs + members.size() >= s->opt.max_inflight) break;
+        const Job &c = s->pending[k];
+        if (c.plan != h.plan || c.req.input ||
+            total + c.req.batch > s->opt.max_batch)
+          continue;
+        members.push_back(k);
+        total += c.req.batch;
+      }
     }
-    if (j.work_bytes && !s->arena.alloc(j.work_bytes, &work_off)) {
-      if (j.in_bytes) s->arena.release(in_off, j.in_bytes);
-      s->arena.release(out_off, j.out_bytes);
-      return;
+    Block b;
+    if (!alloc_block(s, h.req, total, h.plan, &b)) {
+      if (members.size() == 1) return;  /* the head waits for memory */
+      members.resize(1);
+      total = h.req.batch;
+      b = Block();
+      if (!alloc_block(s, h.req, total, h.plan, &b)) return;
     }
-    j.in_off = in_off;
-    j.out_off = out_off;
-    j.work_off = work_off;
+    b.refs = members.size();
+    const uint64_t block_id = s->next_block++;
+    s->blocks[block_id] = b;
     if (s->arena.used() > s->peak_bytes) s->peak_bytes = s->arena.used();
     int best = 0;
     for (size_t k = 1; k < s->streams.size(); ++k)
       if (s->stream_load[k] < s->stream_load[best]) best = (int)k;
-    j.stream = best;
     void *stream = s->streams[best];
-    const void *in = j.req.input ? j.req.input : s->arena_base + j.in_off;
-    void *out = s->arena_base + j.out_off;
-    const size_t count = j.req.length * j.req.batch;
+    const size_t cb = complex_bytes(h.req.precision_bits);
+    const size_t n = h.req.length;
+    Group g;
+    g.stream = best;
     int rc = 0;
-    if (!j.req.input)
-      rc = mofft_hip_fill_uniform((void *)in, j.req.precision_bits,
-                                  (int64_t)count, j.req.seed, stream);
-    void *work = j.work_bytes ? s->arena_base + j.work_off : nullptr;
+    size_t offset = 0;  /* transforms before this member */
+    const double start = now_seconds();
+    for (size_t m : members) {
+      Job j = s->pending[m];
+      j.block = block_id;
+      j.start = start;
+      j.out_off = b.out_off + offset * n * cb;
+      j.out_bytes = j.req.batch * n * cb;
+      if (!j.req.input && rc == 0)
+        rc = mofft_hip_fill_uniform(s->arena_base + b.in_off + offset * n * cb,
+                                    j.req.precision_bits,
+                                    (int64_t)(j.req.batch * n), j.req.seed,
+                                    stream);
+      offset += j.req.batch;
+      g.jobs.push_back(j);
+    }
+    /* (h refers into `pending`: not used past this point) */
+    for (size_t k = members.size(); k-- > 0;)
+      s->pending.erase(s->pending.begin() + (ptrdiff_t)members[k]);
+    const Job &head = g.jobs[0];
+    const void *in = head.req.input ? head.req.input : s->arena_base + b.in_off;
+    void *out = s->arena_base + b.out_off;
+    void *work = b.work_bytes ? s->arena_base + b.work_off : nullptr;
     mofft_status st = rc == 0 ? mofft_internal_execute_count(
-                                    j.plan, in, out, work, j.req.batch, stream)
+                                    head.plan, in, out, work, total, stream)
                               : MOFFT_INTERNAL_ERROR;
-    if (s->free_events.empty()) {
+    if (st == MOFFT_SUCCESS && s->free_events.empty()) {
       void *ev = nullptr;
       if (mofft_hip_event_create(&ev) != 0) st = MOFFT_INTERNAL_ERROR;
       else s->free_events.push_back(ev);
     }
-    j.start = now_seconds();
     if (st == MOFFT_SUCCESS) {
-      j.event = s->free_events.back();
+      g.event = s->free_events.back();
       s->free_events.pop_back();
-      if (mofft_hip_event_record(j.event, stream) != 0) st = MOFFT_INTERNAL_ERROR;
+      if (mofft_hip_event_record(g.event, stream) != 0) {
+        s->free_events.push_back(g.event);
+        st = MOFFT_INTERNAL_ERROR;
+      }
     }
+    s->executions++;
     if (st != MOFFT_SUCCESS) {
-      if (j.event) s->free_events.push_back(j.event);
-      j.event = nullptr;
-      s->completed++;
-      s->ready.push_back(completion_of(s, j, now_seconds(), st));
-      s->done[j.ticket] = j;
-      s->pending.pop_front();
+      const double t = now_seconds();
+      for (const Job &j : g.jobs) finish_job(s, j, t, st);
       continue;
     }
     s->stream_load[best]++;
-    s->inflight.push_back(j);
-    if (s->inflight.size() > s->peak_inflight) s->peak_inflight = s->inflight.size();
-    s->pending.pop_front();
+    s->inflight_jobs += g.jobs.size();
+    s->inflight.push_back(std::move(g));
+    if (s->inflight_jobs > s->peak_inflight) s->peak_inflight = s->inflight_jobs;
   }
 }
 
@@ -307,7 +394,7 @@ mofft_status do_submit(mofft_server *s, const mofft_request *r, double t,
   mofft_status st = get_plan(s, *r, &j.plan);
   if (st != MOFFT_SUCCESS) return st;
   size_t in_b = 0, out_b = 0, work_b = 0;
-  if (request_bytes(*r, j.plan, &in_b, &out_b, &work_b) > s->arena.size())
+  if (group_bytes(*r, r->batch, j.plan, &in_b, &out_b, &work_b) > s->arena.size())
     return MOFFT_ALLOCATION_FAILURE;
   j.ticket = s->next_ticket++;
   s->pending.push_back(j);
@@ -333,7 +420,8 @@ mofft_status mofft_server_create(mofft_server **server,
   mofft_server_options o;
   if (options) o = *options;
   else mofft_server_options_init(&o);
-  if (o.budget_bytes < Arena::kAlign || o.flags != 0) return MOFFT_INVALID_ARGUMENT;
+  if (o.budget_bytes < Arena::kAlign || (o.flags & ~MOFFT_SERVER_NO_COALESCE) != 0)
+    return MOFFT_INVALID_ARGUMENT;
   if (o.streams == 0) o.streams = 4;
   if (o.max_inflight == 0) o.max_inflight = 4 * o.streams;
   if (o.max_batch == 0) o.max_batch = 64;
@@ -404,7 +492,7 @@ mofft_status mofft_server_get_stats(const mofft_server *s,
   st->submitted = s->submitted;
   st->completed = s->completed;
   st->pending = s->pending.size();
-  st->inflight = s->inflight.size();
+  st->inflight = s->inflight_jobs;
   st->resident_plans = s->plans.size();
   st->bytes_in_use = s->arena.used();
   st->peak_bytes_in_use = s->peak_bytes;
@@ -412,6 +500,7 @@ mofft_status mofft_server_get_stats(const mofft_server *s,
   st->peak_inflight = s->peak_inflight;
   st->flops_completed = s->flops;
   st->bytes_completed = s->bytes;
+  st->executions = s->executions;
   return MOFFT_SUCCESS;
 }
 
@@ -479,7 +568,7 @@ void mofft_server_destroy(mofft_server *s) {
   if (!s) return;
   for (void *st : s->streams) mofft_hip_synchronize(st);
   for (auto &kv : s->plans) destroy_plan(kv.first.bits, kv.second);
-  for (auto &j : s->inflight) mofft_hip_event_destroy(j.event);
+  for (auto &g : s->inflight) mofft_hip_event_destroy(g.event);
   for (void *ev : s->free_events) mofft_hip_event_destroy(ev);
   for (void *st : s->streams) mofft_hip_stream_destroy(st);
   mofft_hip_free(s->arena_base);

@@ -195,3 +195,46 @@ def test_budget_serialises_large_requests(gpu, hip_copy):
         assert st.peak_bytes_in_use <= budget and st.peak_inflight == 2
     x = seeded_device_uniform(9, n * b, 64).reshape(b, n)
     assert oracle.rel_l2(done[2], oracle.dft(x, -1)) <= tol(n, 64)
+
+
+def test_coalesced_requests(gpu, hip_copy):
+    """Pending requests of one plan share executions (continuous batching):
+    fewer executions than requests, and every result bit-identical to the
+    server that runs each request alone, and within the oracle bar."""
+    n = 4096
+    reqs = [(n, 1 + i % 3, 32, gpu.FORWARD, 500 + i, i) for i in range(24)]
+    reqs += [(n, 2, 32, gpu.BACKWARD, 900 + i, 24 + i) for i in range(4)]
+
+    def serve(coalesce):
+        out = {}
+        with gpu.Server(256 << 20, streams=2, max_inflight=6, max_batch=16,
+                        coalesce=coalesce) as srv:
+            for (nn, bb, bits, d, seed, tag) in reqs:
+                srv.submit(nn, bb, bits, d, seed=seed, tag=tag)
+            while len(out) < len(reqs):
+                for c in srv.poll(wait=True):
+                    assert c.status == gpu.SUCCESS
+                    out[c.tag] = hip_copy(c.output, n, reqs[c.tag][1], 32)
+                    srv.release(c.ticket)
+            st = srv.stats()
+            assert st.bytes_in_use == 0 and st.completed == len(reqs)
+            assert st.peak_inflight <= 6
+        return out, st
+
+    alone, sa = serve(False)
+    mixed, sb = serve(True)
+    assert sa.executions == len(reqs)
+    assert sb.executions < len(reqs) // 2
+    for (nn, bb, bits, d, seed, tag) in reqs:
+        assert np.array_equal(alone[tag], mixed[tag]), tag
+        x = seeded_device_uniform(seed, n * bb, 32).reshape(bb, n).astype(np.complex128)
+        assert oracle.rel_l2(mixed[tag], oracle.dft(x, d)) <= tol(n, 32), tag
+
+
+def test_coalescing_flag_rejects_unknown_bits(gpu):
+    import ctypes as C
+    o = gpu.ServerOptions()
+    gpu.lib().mofft_server_options_init(C.byref(o))
+    o.flags = 4
+    h = C.c_void_p()
+    assert gpu.lib().mofft_server_create(C.byref(h), C.byref(o)) == gpu.INVALID_ARGUMENT
