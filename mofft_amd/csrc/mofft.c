@@ -897,8 +897,11 @@ static void enumerate_splits(int precision_bits, size_t remaining,
   if (cur->count >= max_passes) return;
   for (int i = 0; i < mofft_registry_count; ++i) {
     const mofft_registry_entry *e = &mofft_registry[i];
-    if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
-      continue;
+    if (e->precision_bits != precision_bits || e->variant != 0) continue;
+    /* row kernels are whole single-pass transforms; column kernels are the
+     * passes of a split (and single-pass chains where a row kernel with
+     * the same stages exists) */
+    if (e->rows && !(cur->count == 0 && (size_t)e->length == length)) continue;
     const size_t l = (size_t)e->length;
     if (l < 2 || remaining % l) continue;
     cur->lengths[cur->count] = l;
