@@ -10,9 +10,19 @@
  * phase, and records the faster one.  The JSON goes through
  * mofft_compiler.wisdom into the library at the next build.
  *
+ * MEASURE times at most 64 candidates ranked by the cost model (the option
+ * limit of the reference); the model can rank a fast kernel low.  So the
+ * tuner also sweeps the kernel registry itself: every single-pass row
+ * kernel of the length, every split into two column-kernel passes whose
+ * lengths multiply to it (and, above 2^20 points, into three passes of at
+ * least 64 points), each under every plain tile bank, timed the same way;
+ * the fastest of the MEASURE winner, the sweep winner and the model plan is
+ * recorded.
+ *
  * usage: mofft_tune [--precision 32|64|both] [--lengths N,N,...]
  *                   [--min-log2 A --max-log2 B] [--points P]
- *                   [--candidates C] [--samples S] -o wisdom.json
+ *                   [--candidates C] [--samples S] [--no-sweep]
+ *                   -o wisdom.json
  */
 #include <hip/hip_runtime.h>
 
@@ -87,6 +97,110 @@ double steady(int bits, void *p, void *in, void *out, hipStream_t s, int reps) {
   return t[t.size() / 2];
 }
 
+/* A chain (outer -> inner) and pass mask built from registry kernels. */
+struct Chain {
+  std::vector<int> radices;
+  uint64_t mask;
+  bool operator==(const Chain &o) const {
+    return radices == o.radices && mask == o.mask;
+  }
+};
+
+struct Kernel {
+  int length;
+  std::vector<int> stages;  /* execution order */
+};
+
+/* Distinct (length, stages) kernels of one precision and kind. */
+std::vector<Kernel> registry_kernels(int bits, bool rows) {
+  std::vector<Kernel> out;
+  for (int i = 0; i < mofft_kernel_count(); ++i) {
+    mofft_kernel_info k;
+    if (mofft_kernel_get_info(i, &k) != MOFFT_SUCCESS) continue;
+    if (k.precision_bits != bits || (k.rows != 0) != rows) continue;
+    Kernel e{k.length, std::vector<int>(k.stages, k.stages + k.stage_count)};
+    bool seen = false;
+    for (const Kernel &o : out) seen |= o.length == e.length && o.stages == e.stages;
+    if (!seen) out.push_back(e);
+  }
+  return out;
+}
+
+/* Passes in execution order -> chain: the last pass's stages first, each
+ * reversed; a boundary bit after every pass but the first. */
+Chain chain_of(const std::vector<const Kernel *> &passes) {
+  Chain c{{}, 0};
+  for (size_t p = passes.size(); p-- > 0;) {
+    const std::vector<int> &st = passes[p]->stages;
+    for (size_t i = st.size(); i-- > 0;) c.radices.push_back(st[i]);
+    if (p > 0) c.mask |= (uint64_t)1 << (c.radices.size() - 1);
+  }
+  return c;
+}
+
+std::vector<Chain> registry_chains(int bits, size_t n) {
+  std::vector<Chain> out;
+  auto add = [&](const Chain &c) {
+    if (std::find(out.begin(), out.end(), c) == out.end()) out.push_back(c);
+  };
+  for (const Kernel &k : registry_kernels(bits, true))
+    if ((size_t)k.length == n) add(chain_of({&k}));
+  const std::vector<Kernel> cols = registry_kernels(bits, false);
+  for (const Kernel &a : cols) {
+    if (n % (size_t)a.length) continue;
+    for (const Kernel &b : cols)
+      if ((size_t)b.length == n / (size_t)a.length) add(chain_of({&a, &b}));
+  }
+  /* three passes only where two would need tiles of more than 1024
+   * points: passes of >= 64 points, the first two stage chains of each
+   * pass length */
+  if (n <= ((size_t)1 << 20)) return out;
+  std::vector<const Kernel *> c3;
+  for (const Kernel &k : cols) {
+    if (k.length < 64) continue;
+    int same = 0;
+    for (const Kernel *o : c3) same += o->length == k.length;
+    if (same < 2) c3.push_back(&k);
+  }
+  for (const Kernel *a : c3) {
+    if (n % (size_t)a->length) continue;
+    const size_t r = n / (size_t)a->length;
+    for (const Kernel *b : c3) {
+      if (r % (size_t)b->length) continue;
+      for (const Kernel *c : c3)
+        if ((size_t)c->length == r / (size_t)b->length) add(chain_of({a, b, c}));
+    }
+  }
+  return out;
+}
+
+void *plan_with_chain(int bits, size_t n, size_t batch, const Chain &c,
+                      unsigned bank) {
+  void *p = nullptr;
+  mofft_status st;
+  if (bits == 32)
+    st = mofft_plan_create_batched_with_radices_f32(
+        (mofft_plan_f32 **)&p, n, batch, MOFFT_FORWARD, MOFFT_OUT_OF_PLACE,
+        c.radices.data(), c.radices.size());
+  else
+    st = mofft_plan_create_batched_with_radices_f64(
+        (mofft_plan_f64 **)&p, n, batch, MOFFT_FORWARD, MOFFT_OUT_OF_PLACE,
+        c.radices.data(), c.radices.size());
+  if (st != MOFFT_SUCCESS) return nullptr;
+  if (bits == 32)
+    st = mofft_plan_set_blocked_transpose_mask_f32((mofft_plan_f32 *)p, c.mask);
+  else
+    st = mofft_plan_set_blocked_transpose_mask_f64((mofft_plan_f64 *)p, c.mask);
+  if (st == MOFFT_SUCCESS)
+    st = bits == 32 ? mofft_plan_set_kernel_bank_f32((mofft_plan_f32 *)p, bank)
+                    : mofft_plan_set_kernel_bank_f64((mofft_plan_f64 *)p, bank);
+  if (st != MOFFT_SUCCESS) {
+    plan_destroy(bits, p);
+    return nullptr;
+  }
+  return p;
+}
+
 std::vector<size_t> parse_list(const char *s) {
   std::vector<size_t> v;
   while (*s) {
@@ -107,6 +221,7 @@ int main(int argc, char **argv) {
   size_t points = (size_t)1 << 24;
   size_t candidates = 32;
   unsigned samples = 5;
+  bool sweep = true;
   const char *out_path = "wisdom.json";
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -120,6 +235,7 @@ int main(int argc, char **argv) {
     else if (a == "--points") points = strtoull(next(), nullptr, 10);
     else if (a == "--candidates") candidates = strtoull(next(), nullptr, 10);
     else if (a == "--samples") samples = (unsigned)atoi(next());
+    else if (a == "--no-sweep") sweep = false;
     else if (a == "-o") out_path = next();
     else { fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
   }
@@ -157,32 +273,71 @@ int main(int argc, char **argv) {
       }
       mofft_plan_diagnostics dm;
       void *model = plan_create(bits, n, batch, nullptr, &dm, &st);
-      /* interleaved steady timing of winner and model (plan_search.cpp:612-622) */
-      double tw = 0, tm = 0;
+      /* registry sweep: every kernel-backed chain under every plain bank */
+      void *swept = nullptr;
+      Chain swept_chain{{}, 0};
+      unsigned swept_bank = 0;
+      size_t swept_count = 0;
+      if (sweep) {
+        double best = 0;
+        for (const Chain &c : registry_chains(bits, n)) {
+          for (unsigned bank : {0u, 1u, 2u, 4u}) {
+            void *p = plan_with_chain(bits, n, batch, c, bank);
+            if (!p) continue;
+            ++swept_count;
+            const double t = steady(bits, p, in, out, stream, samples);
+            if (!swept || t < best) {
+              if (swept) plan_destroy(bits, swept);
+              swept = p;
+              swept_chain = c;
+              swept_bank = bank;
+              best = t;
+            } else {
+              plan_destroy(bits, p);
+            }
+          }
+        }
+      }
+      /* interleaved steady timing of the finalists (plan_search.cpp:612-622) */
+      double tw = 0, tm = 0, tsw = 0;
       for (int round = 0; round < 3; ++round) {
         tw += steady(bits, measured, in, out, stream, 15);
         if (model) tm += steady(bits, model, in, out, stream, 15);
+        if (swept) tsw += steady(bits, swept, in, out, stream, 15);
       }
       Result r;
       r.length = n;
       r.precision = bits;
       r.batch = batch;
-      r.candidates = d.candidates_measured;
+      r.candidates = d.candidates_measured + swept_count;
       r.model_seconds = tm / 3;
-      const bool use_measured = !model || tw <= tm * 0.99;
-      const mofft_plan_diagnostics &win = use_measured ? d : dm;
-      r.radices.assign(win.radices, win.radices + win.radix_count);
-      r.mask = win.blocked_transpose_mask;
-      r.bank = win.kernel_bank;
-      r.seconds = use_measured ? tw / 3 : tm / 3;
+      /* the model plan unless a measured one is at least 1% faster */
+      const mofft_plan_diagnostics *win = model ? &dm : &d;
+      double tbest = model ? tm : tw;
+      const char *which = model ? "model" : "measured";
+      if (!model || tw <= tbest * 0.99) { win = &d; tbest = tw; which = "measured"; }
+      if (swept && tsw <= tbest * 0.99) {
+        tbest = tsw;
+        which = "swept";
+        r.radices = swept_chain.radices;
+        r.mask = swept_chain.mask;
+        r.bank = swept_bank;
+      } else {
+        r.radices.assign(win->radices, win->radices + win->radix_count);
+        r.mask = win->blocked_transpose_mask;
+        r.bank = win->kernel_bank;
+      }
+      r.seconds = tbest / 3;
       results.push_back(r);
       const double gbs = 2.0 * n * batch * cb / r.seconds / 1e9;
-      printf("f%d N=%zu batch=%zu: %s plan, %.3f us (model %.3f us), %.0f GB/s, %zu candidates\n",
-             bits, n, batch, use_measured ? "measured" : "model", r.seconds * 1e6,
-             r.model_seconds * 1e6, gbs, r.candidates);
+      printf("f%d N=%zu batch=%zu: %s plan, %.3f us (model %.3f us, measured %.3f us, "
+             "swept %.3f us), %.0f GB/s, %zu candidates\n",
+             bits, n, batch, which, r.seconds * 1e6, r.model_seconds * 1e6, tw / 3 * 1e6,
+             tsw / 3 * 1e6, gbs, r.candidates);
       fflush(stdout);
       plan_destroy(bits, measured);
       if (model) plan_destroy(bits, model);
+      if (swept) plan_destroy(bits, swept);
       hipFree(in);
       hipFree(out);
     }
