@@ -477,8 +477,8 @@ def run_sweep(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
-              "batched transforms shard across GPUs with no exchange, see DESIGN.md",
+        print(f"bench.py: runs on one GPU only (--gpus {args.gpus} requested); "
+              "batched transforms would shard across GPUs with no exchange",
               file=sys.stderr)
         sys.exit(2)
     line, shape = run_gpu(args)

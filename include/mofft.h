@@ -191,8 +191,8 @@ mofft_status mofft_plan_get_kernel_bank_f32(const mofft_plan_f32 *plan,
 mofft_status mofft_plan_get_kernel_bank_f64(const mofft_plan_f64 *plan,
                                              unsigned int *bank);
 
-/* include/mofft.h:178-201.  Text plan recipes, scoped to this build and to
- * the GPU architecture.  Export with buffer=NULL, capacity=0 to query the
+/* include/mofft.h:178-201.  Text plan recipes, scoped to the library binary
+ * (its build ID) and to the GPU architecture.  Export with buffer=NULL, capacity=0 to query the
  * size (including the final NUL). */
 const char *mofft_build_id(void);
 mofft_status mofft_plan_export_f32(const mofft_plan_f32 *plan, char *buffer,
