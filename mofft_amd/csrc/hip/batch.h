@@ -71,6 +71,7 @@ __device__ __forceinline__ void setup_gather(Ctx<T> &x,
   x.col0 = (uint32_t)block * C;
   const uint32_t col = x.col0 + x.c;
   x.live = col < (uint32_t)p.columns;
+  x.contiguous = false; /* each column its own job's buffer */
   const void *in = nullptr;
   void *out = nullptr;
   uint32_t flags = 0;

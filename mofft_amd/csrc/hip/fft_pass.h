@@ -108,6 +108,9 @@ struct Ctx {
   uint32_t ns;
   uint32_t obase;     /* (j / ns) * ns * L + j % ns */
   uint32_t col0;      /* first column of this tile */
+  uint32_t tvalid;    /* contiguous row tile: elements it covers */
+  const cx<T> *tsrc;  /* ... its first input element */
+  cx<T> *tdst;        /* ... its first output element */
   uint32_t jm_scale;  /* (j % ns) * tw_scale */
   uint32_t split_mask;
   uint32_t split_shift;
@@ -115,6 +118,7 @@ struct Ctx {
   int c;
   int t;
   bool live;
+  bool contiguous;    /* row tile over consecutive transforms of one buffer */
   bool swap_in;
   bool swap_out;
   bool pass_tw;
@@ -329,9 +333,83 @@ struct Stage {
     }
   }
 
+  /* One thread per whole transform (TPT == 1, R == L): a direct load has
+   * lanes a transform apart, 64 cache lines per load instruction.  A
+   * contiguous tile is loaded coalesced instead (element g of the tile by
+   * thread g % C) and handed to its owners through LDS, real parts then
+   * imaginary parts. */
+  static constexpr bool STAGED = ROWS && TPT == 1 && IT == 1;
+
+  static __device__ __forceinline__ void load_staged(const Ctx<T> &x,
+                                                     T (&vr)[IT][R],
+                                                     T (&vi)[IT][R]) {
+    constexpr int NT = C;
+    cx<T> v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint32_t g = (uint32_t)(x.tid + i * NT);
+      v[i].x = T(0);
+      v[i].y = T(0);
+      if (g < x.tvalid) v[i] = ld_stream<T>(x.tsrc + g);
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      if (half) __syncthreads();
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int g = x.tid + i * NT;
+        const int c = g / L;
+        const bool re = (half == 0) != x.swap_in;
+        x.lds[TL::row_addr(c, g - c * L)] = re ? v[i].x : v[i].y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int e = 0; e < R; ++e) {
+        const T w = x.lds[TL::row_addr(x.tid, e)];
+        if (half) vi[0][e] = w;
+        else vr[0][e] = w;
+      }
+    }
+    __syncthreads();
+  }
+
+  static __device__ __forceinline__ void store_staged(const Ctx<T> &x,
+                                                      T (&vr)[IT][R],
+                                                      T (&vi)[IT][R]) {
+    constexpr int NT = C;
+    T outr[R], outi[R];
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      __syncthreads();
+#pragma unroll
+      for (int o = 0; o < R; ++o)
+        x.lds[TL::row_addr(x.tid, o)] = half ? vi[0][o] : vr[0][o];
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < R; ++i) {
+        const int g = x.tid + i * NT;
+        const int c = g / L;
+        const T w = x.lds[TL::row_addr(c, g - c * L)];
+        if (half) outi[i] = w;
+        else outr[i] = w;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+      const uint32_t g = (uint32_t)(x.tid + i * NT);
+      if (g < x.tvalid) store_global<T>(x.tdst + g, outr[i], outi[i], x.swap_out);
+    }
+  }
+
   static __device__ __forceinline__ void load_first(const Ctx<T> &x,
                                                     T (&vr)[IT][R],
                                                     T (&vi)[IT][R]) {
+    if constexpr (STAGED) {
+      if (x.contiguous) {
+        load_staged(x, vr, vi);
+        return;
+      }
+    }
     if constexpr (MOFFT_VEC_SMALL && ROWS && TPT == 1 && IT == 1 &&
                   (R % 2 == 0) && sizeof(T) == 4) {
       /* a whole small transform per thread: 16-byte loads */
@@ -449,6 +527,12 @@ struct Stage {
   static __device__ __forceinline__ void store_direct(const Ctx<T> &x,
                                                       T (&vr)[IT][R],
                                                       T (&vi)[IT][R]) {
+    if constexpr (STAGED) {
+      if (x.contiguous) {
+        store_staged(x, vr, vi);
+        return;
+      }
+    }
     if (!x.live) return;
     if constexpr (MOFFT_VEC_SMALL && ROWS && TPT == 1 && IT == 1 &&
                   (R % 2 == 0) && sizeof(T) == 4) {
@@ -540,6 +624,13 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.col0 = (uint32_t)block * C;
   const uint32_t col = x.col0 + x.c;
   x.live = col < (uint32_t)p.columns;
+  x.contiguous = ROWS;
+  if (ROWS) {
+    const uint32_t cols = (uint32_t)p.columns - x.col0;
+    x.tvalid = (cols < (uint32_t)C ? cols : (uint32_t)C) * (uint32_t)L;
+    x.tsrc = (const cx<T> *)p.in + x.col0 * (uint32_t)p.n;
+    x.tdst = (cx<T> *)p.out + x.col0 * (uint32_t)p.n;
+  }
   const uint32_t b = ROWS ? col : div_by(col, p.stride_magic, p.stride_shift);
   const uint32_t j = ROWS ? 0u : col - b * (uint32_t)p.stride;
   x.stride = (uint32_t)p.stride;
@@ -663,7 +754,8 @@ struct PassKernel {
     if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
     /* LDS is only touched by sub-stage exchanges and by the transposed
      * store of strided passes whose output interleaves < C columns */
-    const bool needs_lds = STAGES > 1 || HAS_MFMA || (!ROWS && a->ns < C);
+    const bool needs_lds =
+        STAGES > 1 || HAS_MFMA || (!ROWS && a->ns < C) || (ROWS && TPT == 1);
     hipLaunchKernelGGL((fft_pass_kernel<T, L, TPT, C, ROWS, R...>),
                        dim3((unsigned)blocks), dim3(THREADS),
                        needs_lds ? LDS : 0, s, *a);
