@@ -115,3 +115,21 @@ def test_explicit_chains(gpu, chain, bits):
     ref = reference(x.astype(np.complex64 if bits == 32 else np.complex128), -1)
     err = oracle.rel_l2(y, ref)
     assert err <= tol(n, bits), (chain, bits, err, desc.passes)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 12, 16])
+@pytest.mark.parametrize("batch", [1, 127, 128, 129, 300])
+def test_small_transform_tiles(gpu, n, bits, batch):
+    """Lengths whose whole transform is one thread's work: the tile of 128
+    transforms is read and written coalesced and handed over through LDS,
+    so full, partial and single-transform tiles, both directions and both
+    placements all have to agree with the oracle."""
+    x = seeded_complex((batch, n), seed=7000 + 13 * n + batch)
+    xr = x.astype(np.complex64 if bits == 32 else np.complex128)
+    for direction, sign in ((gpu.FORWARD, -1), (gpu.BACKWARD, 1)):
+        ref = oracle.dft(xr, sign)
+        for in_place in (False, True):
+            y, desc = run_host(gpu, n, batch, bits, direction, in_place, x)
+            err = oracle.rel_l2(y, ref)
+            assert err <= tol(n, bits), (n, batch, bits, direction, in_place, err)
