@@ -34,6 +34,11 @@ STAGE_RADICES = (16, 15, 14, 13, 12, 11, 10, 9, 8, 7, 6, 5, 4, 3, 2)
 # four-step factors
 MIXED_ROWS = (30, 45, 121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
+# powers of the small odd and composite radices up to 4096 points, so that a
+# length such as 3^10 or 10^5 gets a two-pass schedule (729 x 81, 1000 x 100)
+# instead of one pass per radix stage
+SMALL_POWERS = (25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729,
+                1000, 1296, 2187, 2401, 3125)
 # Besides the radix-16 chains, every power-of-two length from 64 has kernels
 # with radix-8 and with radix-4 sub-stages: half (a quarter)
 # of the registers per butterfly - an FP64 radix-16 butterfly holds 64 VGPRs
@@ -247,7 +252,7 @@ def entries() -> list[Entry]:
             add(_make(precision, length, 1, _pow2_chain(length)))
         for r in STAGE_RADICES:
             add(_make(precision, r, 1, (r,)))
-        for length in MIXED_ROWS:
+        for length in MIXED_ROWS + SMALL_POWERS:
             add(_make(precision, length, 1, chain_for(length)))
         for k in range(7, rows_max.bit_length()):
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
@@ -263,7 +268,7 @@ def entries() -> list[Entry]:
             add(_make(precision, length, 0, _pow2_chain(length)))
         for r in STAGE_RADICES:
             add(_make(precision, r, 0, (r,)))
-        for length in MIXED_COLS:
+        for length in MIXED_COLS + SMALL_POWERS:
             add(_make(precision, length, 0, chain_for(length)))
         for k in range(7, cols_max.bit_length()):
             add(_make(precision, 1 << k, 0, _pow2_chain8(1 << k)))

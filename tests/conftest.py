@@ -78,3 +78,26 @@ def seeded_device_uniform(seed, count, bits):
         v = v.astype(np.float32)
         return v.view(np.complex64)
     return v.view(np.complex128)
+
+
+# powers of the small radices with kernels of their own (3^k, 5^k, 6^k, 7^k,
+# 10^k up to 4096 points)
+SMALL_POWERS = [25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729, 1000,
+                1296, 2187, 2401, 3125]
+
+
+def small_power_plans(mofft, bits):
+    """(kernel index, N, outer-to-inner radices, pass mask) for every
+    compiled kernel of a SMALL_POWERS length: a whole-transform kernel runs
+    as the single pass of N = L, a strided kernel as the first pass of
+    N = 16 L (radix 16 outermost, the boundary right after it)."""
+    out = []
+    for i, k in enumerate(mofft.kernels()):
+        if k.precision_bits != bits or k.length not in SMALL_POWERS:
+            continue
+        inner = list(reversed(k.stages[: k.stage_count]))
+        if k.rows:
+            out.append((i, k.length, inner, 0))
+        else:
+            out.append((i, 16 * k.length, [16] + inner, 1))
+    return out

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import seeded_complex
+from conftest import seeded_complex, small_power_plans
 
 pytestmark = pytest.mark.gpu
 
@@ -133,3 +133,23 @@ def test_small_transform_tiles(gpu, n, bits, batch):
             y, desc = run_host(gpu, n, batch, bits, direction, in_place, x)
             err = oracle.rel_l2(y, ref)
             assert err <= tol(n, bits), (n, batch, bits, direction, in_place, err)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_small_power_kernels(gpu, bits):
+    """Each kernel of a small-radix power length (3^k, 5^k, 6^k, 7^k, 10^k
+    up to 4096 points; chains of radix 3, 5, 6, 7, 9, 10 stages) against the
+    oracle, forward out of place and backward in place."""
+    plans = small_power_plans(gpu, bits)
+    assert plans
+    for i, n, radices, mask in plans:
+        batch = max(1, min(8, 8192 // n))
+        x = seeded_complex((batch, n), seed=9100 + n + bits)
+        xr = x.astype(np.complex64 if bits == 32 else np.complex128)
+        for direction, sign, in_place in ((gpu.FORWARD, -1, False),
+                                          (gpu.BACKWARD, 1, True)):
+            y, desc = run_host(gpu, n, batch, bits, direction, in_place, x,
+                               radices=radices, mask=mask)
+            assert desc.pass_kernel[0] == i
+            err = oracle.rel_l2(y, reference(xr, sign))
+            assert err <= tol(n, bits), (n, radices, mask, direction, err)

@@ -16,6 +16,7 @@ from pathlib import Path
 import pytest
 
 import mofft_amd as mf
+from conftest import SMALL_POWERS, small_power_plans
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -105,8 +106,9 @@ def test_no_device_means_unsupported_hardware():
 
 @pytest.mark.parametrize("n,bits,passes", [
     (1 << 25, 32, [512, 256, 256]), (1 << 25, 64, [512, 256, 256]),
-    (45, 32, [45]), (20736, 64, [144, 144]), (371293, 32, [2197, 169]),
-    (1728, 64, [1728]), (759375, 32, [3375, 225]),
+    (45, 32, [45]), (1000, 64, [1000]), (59049, 32, [243, 243]),
+    (117649, 64, [343, 343]), (100000, 32, [1000, 100]),
+    (161051, 64, [121, 121, 11]), (8748, 64, [243, 36]),
 ])
 def test_model_schedule(n, bits, passes):
     """Lengths without a wisdom entry: the analytical planner's schedule
@@ -116,6 +118,19 @@ def test_model_schedule(n, bits, passes):
     assert math.prod(d.chain) == n
     # the mask marks exactly the pass boundaries
     assert bin(d.pass_mask).count("1") == len(passes) - 1
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_small_power_kernels_selected(bits):
+    """Every kernel of a small-radix power length is the one its plan runs
+    (whole-transform kernels as the single pass, strided kernels as the first
+    pass), so the GPU parity test over these plans covers each of them."""
+    plans = small_power_plans(mf, bits)
+    assert {mf.kernels()[i].length for i, *_ in plans} == set(SMALL_POWERS)
+    for i, n, radices, mask in plans:
+        d = mf.describe(n, 1, bits, radices, mask)
+        assert d.pass_kernel[0] == i, (n, radices, mask)
+        assert d.passes == ([n] if mask == 0 else [n // 16, 16])
 
 
 @pytest.mark.parametrize("bits", [32, 64])
