@@ -184,7 +184,8 @@ __device__ __forceinline__ void store_global(cx<T> *p, T a, T b, bool swap) {
 }
 
 /* Cooperative store of the whole tile, used when the pass output of
- * adjacent columns is not contiguous (ns < C).  Each thread owns the tile
+ * adjacent columns is not contiguous (ns < C, C a multiple of ns: the tile
+ * is C / ns whole groups, each one contiguous block).  Each thread owns the tile
  * elements q = tid + i*NT; real and imaginary parts pass through LDS in
  * turn (row layout). */
 template <typename T, int L, int TPT, int C, bool ROWS, int IT, int R>
@@ -274,6 +275,18 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
       store_global<T>(out + pos, outr[i], outi[i], swap);
     }
   }
+}
+
+/* The cooperative tile store enumerates whole output groups of ns adjacent
+ * columns (one contiguous run of ns*L elements each): it needs ns < C and C
+ * a multiple of ns, so that a tile is whole groups.  Otherwise (ns = 12
+ * under 256 columns, say) each thread stores its own column, runs of ns
+ * elements across adjacent lanes. */
+template <int C>
+__device__ __forceinline__ bool tile_store(const mofft_pass_args &p) {
+  const uint32_t ns = (uint32_t)p.ns;
+  return ns < (uint32_t)C &&
+         div_by((uint32_t)C, p.ns_magic, p.ns_shift) * ns == (uint32_t)C;
 }
 
 /* Radix value that marks a radix-16 sub-stage evaluated on the matrix
@@ -607,7 +620,7 @@ struct Stage {
 #endif
       Next::run(x, p, nr, ni);
     } else {
-      if (ROWS || p.ns >= C) store_direct(x, vr, vi);
+      if (ROWS || !tile_store<C>(p)) store_direct(x, vr, vi);
       else store_tile<T, L, TPT, C, ROWS, IT, R>(x, p, vr, vi);
     }
   }
@@ -964,7 +977,7 @@ struct MStage {
       __syncthreads();
       V::load_lds(x, x.lds, ui);
       __syncthreads();
-      if (ROWS || p.ns >= C) V::store_direct(x, ur, ui);
+      if (ROWS || !tile_store<C>(p)) V::store_direct(x, ur, ui);
       else store_tile<T, L, TPT, C, ROWS, V::IT, 16>(x, p, ur, ui);
     }
   }

@@ -168,11 +168,11 @@ def test_measure_planner(gpu, n, batch, in_place, bits):
 
 
 def test_strict_wisdom_and_model_fallback(gpu):
-    # 1728 has no tuned entry: STRICT fails, the default falls back to the model
+    # 1000 has no tuned entry: STRICT fails, the default falls back to the model
     with pytest.raises(gpu.MofftError) as e:
-        gpu.Plan(1728, 1, 32, flags=gpu.STRICT_WISDOM)
+        gpu.Plan(1000, 1, 32, flags=gpu.STRICT_WISDOM)
     assert e.value.status == gpu.WISDOM_NOT_FOUND
-    with gpu.Plan(1728) as p:
+    with gpu.Plan(1000) as p:
         assert p.diagnostics.used_model_fallback == 1 and p.diagnostics.used_wisdom == 0
     # 1024 has one: STRICT succeeds and reports the wisdom plan
     with gpu.Plan(1024, 1, 32, flags=gpu.STRICT_WISDOM) as p:

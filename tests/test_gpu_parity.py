@@ -153,3 +153,27 @@ def test_small_power_kernels(gpu, bits):
             assert desc.pass_kernel[0] == i
             err = oracle.rel_l2(y, reference(xr, sign))
             assert err <= tol(n, bits), (n, radices, mask, direction, err)
+
+
+# plans whose strided tiles are not whole output groups (C columns per tile
+# not a multiple of the previous passes' product ns): 2-, 3- and 4-pass model
+# plans of smooth lengths, the tuned 12^5 plan, and both pass orders of 16 x 36
+GROUP_CASES = [(18, None, None), (20, None, None), (24, None, None),
+               (35, None, None), (280, None, None), (315, None, None),
+               (420, None, None), (33600, None, None), (67200, None, None),
+               (134400, None, None), (198450, None, None), (248832, None, None),
+               (576, [16, 6, 6], 1), (576, [6, 6, 16], 2), (1296, [6, 6, 6, 6], 2)]
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n,radices,mask", GROUP_CASES)
+def test_partial_output_groups(gpu, n, radices, mask, bits):
+    batch = max(2, min(8, (1 << 16) // n))
+    x = seeded_complex((batch, n), seed=9300 + n)
+    xr = x.astype(np.complex64 if bits == 32 else np.complex128)
+    for direction, sign, in_place in ((gpu.FORWARD, -1, False),
+                                      (gpu.BACKWARD, 1, True)):
+        y, desc = run_host(gpu, n, batch, bits, direction, in_place, x,
+                           radices=radices, mask=mask)
+        err = oracle.rel_l2(y, reference(xr, sign))
+        assert err <= tol(n, bits), (n, desc.passes, direction, err)
