@@ -39,6 +39,23 @@ MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 # instead of one pass per radix stage
 SMALL_POWERS = (25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729,
                 1000, 1296, 2187, 2401, 3125)
+SMOOTH_MAX = 1024
+
+
+def smooth_lengths(limit: int = SMOOTH_MAX) -> tuple[int, ...]:
+    """Every length up to `limit` whose prime factors are radices (2, 3, 5,
+    7, 11, 13): each gets a single-pass and a strided kernel, so a smooth
+    transform of up to 1024 points is one HBM pass and one of up to about a
+    million points two."""
+    out = []
+    for n in range(2, limit + 1):
+        m = n
+        for q in (2, 3, 5, 7, 11, 13):
+            while m % q == 0:
+                m //= q
+        if m == 1:
+            out.append(n)
+    return tuple(out)
 # Besides the radix-16 chains, every power-of-two length from 64 has kernels
 # with radix-8 and with radix-4 sub-stages: half (a quarter)
 # of the registers per butterfly - an FP64 radix-16 butterfly holds 64 VGPRs
@@ -252,7 +269,7 @@ def entries() -> list[Entry]:
             add(_make(precision, length, 1, _pow2_chain(length)))
         for r in STAGE_RADICES:
             add(_make(precision, r, 1, (r,)))
-        for length in MIXED_ROWS + SMALL_POWERS:
+        for length in MIXED_ROWS + SMALL_POWERS + smooth_lengths():
             add(_make(precision, length, 1, chain_for(length)))
         for k in range(7, rows_max.bit_length()):
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
@@ -268,7 +285,7 @@ def entries() -> list[Entry]:
             add(_make(precision, length, 0, _pow2_chain(length)))
         for r in STAGE_RADICES:
             add(_make(precision, r, 0, (r,)))
-        for length in MIXED_COLS + SMALL_POWERS:
+        for length in MIXED_COLS + SMALL_POWERS + smooth_lengths():
             add(_make(precision, length, 0, chain_for(length)))
         for k in range(7, cols_max.bit_length()):
             add(_make(precision, 1 << k, 0, _pow2_chain8(1 << k)))

@@ -17,7 +17,8 @@
  * lengths multiply to it (and, above 2^20 points, into three passes of at
  * least 64 points), each under every plain tile bank, timed the same way;
  * the fastest of the MEASURE winner, the sweep winner and the model plan is
- * recorded.
+ * recorded, after its output for a seeded input has been checked against
+ * the model plan's (relative L2 within four times the precision's bound).
  *
  * usage: mofft_tune [--precision 32|64|both] [--lengths N,N,...]
  *                   [--min-log2 A --max-log2 B] [--points P]
@@ -95,6 +96,36 @@ double steady(int bits, void *p, void *in, void *out, hipStream_t s, int reps) {
   hipEventDestroy(b);
   std::sort(t.begin(), t.end());
   return t[t.size() / 2];
+}
+
+/* Relative L2 distance between two plans' outputs for one seeded input
+ * (at most the first 2^20 points of the batch), so that a timing never
+ * stands in for a result: a plan is recorded only when it agrees with the
+ * analytical plan. */
+double disagreement(int bits, void *a, void *b, void *in, void *out, void *out2,
+                    size_t n, size_t batch, hipStream_t s) {
+  const size_t cb = bits == 32 ? 8 : 16;
+  if (mofft_server_generate(in, bits, n * batch, 2027, s) != MOFFT_SUCCESS ||
+      run(bits, a, in, out, s) != MOFFT_SUCCESS ||
+      run(bits, b, in, out2, s) != MOFFT_SUCCESS ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return INFINITY;
+  const size_t count = std::min(n * batch, std::max(n, (size_t)1 << 20));
+  std::vector<double> x(2 * count), y(2 * count);
+  std::vector<unsigned char> hx(count * cb), hy(count * cb);
+  if (hipMemcpy(hx.data(), out, count * cb, hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(hy.data(), out2, count * cb, hipMemcpyDeviceToHost) != hipSuccess)
+    return INFINITY;
+  for (size_t i = 0; i < 2 * count; ++i) {
+    x[i] = bits == 32 ? ((const float *)hx.data())[i] : ((const double *)hx.data())[i];
+    y[i] = bits == 32 ? ((const float *)hy.data())[i] : ((const double *)hy.data())[i];
+  }
+  double num = 0, den = 0;
+  for (size_t i = 0; i < 2 * count; ++i) {
+    num += (x[i] - y[i]) * (x[i] - y[i]);
+    den += y[i] * y[i];
+  }
+  return den > 0 ? std::sqrt(num / den) : INFINITY;
 }
 
 /* A chain (outer -> inner) and pass mask built from registry kernels. */
@@ -247,13 +278,15 @@ int main(int argc, char **argv) {
   }
   hipStream_t stream = nullptr;
   std::vector<Result> results;
+  size_t mismatches = 0;
   for (int bits : precisions) {
     const size_t cb = bits == 32 ? 8 : 16;
     for (size_t n : lengths) {
       const size_t batch = std::max<size_t>(1, points / n);
-      void *in = nullptr, *out = nullptr;
+      void *in = nullptr, *out = nullptr, *out2 = nullptr;
       if (hipMalloc(&in, n * batch * cb) != hipSuccess ||
-          hipMalloc(&out, n * batch * cb) != hipSuccess) {
+          hipMalloc(&out, n * batch * cb) != hipSuccess ||
+          hipMalloc(&out2, n * batch * cb) != hipSuccess) {
         fprintf(stderr, "allocation failed for N=%zu\n", n);
         return 1;
       }
@@ -268,7 +301,7 @@ int main(int argc, char **argv) {
       void *measured = plan_create(bits, n, batch, &o, &d, &st);
       if (st != MOFFT_SUCCESS) {
         fprintf(stderr, "N=%zu f%d: %s\n", n, bits, mofft_status_string(st));
-        hipFree(in); hipFree(out);
+        hipFree(in); hipFree(out); hipFree(out2);
         continue;
       }
       mofft_plan_diagnostics dm;
@@ -315,10 +348,17 @@ int main(int argc, char **argv) {
       const mofft_plan_diagnostics *win = model ? &dm : &d;
       double tbest = model ? tm : tw;
       const char *which = model ? "model" : "measured";
-      if (!model || tw <= tbest * 0.99) { win = &d; tbest = tw; which = "measured"; }
+      void *chosen = model ? model : measured;
+      if (!model || tw <= tbest * 0.99) {
+        win = &d;
+        tbest = tw;
+        which = "measured";
+        chosen = measured;
+      }
       if (swept && tsw <= tbest * 0.99) {
         tbest = tsw;
         which = "swept";
+        chosen = swept;
         r.radices = swept_chain.radices;
         r.mask = swept_chain.mask;
         r.bank = swept_bank;
@@ -328,6 +368,20 @@ int main(int argc, char **argv) {
         r.bank = win->kernel_bank;
       }
       r.seconds = tbest / 3;
+      if (model && chosen != model) {
+        const double tol = (bits == 32 ? 1e-6 : 1e-15) * std::max(1.0, std::log2((double)n));
+        const double dis = disagreement(bits, chosen, model, in, out, out2, n, batch, stream);
+        if (!(dis <= 4 * tol)) {
+          printf("f%d N=%zu: the %s plan disagrees with the model plan (relative L2 "
+                 "%.3e > %.1e); no entry recorded\n", bits, n, which, dis, 4 * tol);
+          ++mismatches;
+          plan_destroy(bits, measured);
+          plan_destroy(bits, model);
+          if (swept) plan_destroy(bits, swept);
+          hipFree(in); hipFree(out); hipFree(out2);
+          continue;
+        }
+      }
       results.push_back(r);
       const double gbs = 2.0 * n * batch * cb / r.seconds / 1e9;
       printf("f%d N=%zu batch=%zu: %s plan, %.3f us (model %.3f us, measured %.3f us, "
@@ -340,6 +394,7 @@ int main(int argc, char **argv) {
       if (swept) plan_destroy(bits, swept);
       hipFree(in);
       hipFree(out);
+      hipFree(out2);
     }
   }
   FILE *f = fopen(out_path, "w");
@@ -361,5 +416,6 @@ int main(int argc, char **argv) {
   fprintf(f, " ]\n}\n");
   fclose(f);
   printf("wrote %zu entries to %s\n", results.size(), out_path);
-  return 0;
+  if (mismatches) printf("%zu plans disagreed with the model plan\n", mismatches);
+  return mismatches ? 3 : 0;
 }

@@ -80,20 +80,20 @@ def seeded_device_uniform(seed, count, bits):
     return v.view(np.complex128)
 
 
-# powers of the small radices with kernels of their own (3^k, 5^k, 6^k, 7^k,
-# 10^k up to 4096 points)
-SMALL_POWERS = [25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729, 1000,
-                1296, 2187, 2401, 3125]
+# powers of the small radices with kernels of their own beyond 1024 points
+SMALL_POWERS = [1296, 2187, 2401, 3125]
 
 
-def small_power_plans(mofft, bits):
+def mixed_kernel_plans(mofft, bits, rows):
     """(kernel index, N, outer-to-inner radices, pass mask) for every
-    compiled kernel of a SMALL_POWERS length: a whole-transform kernel runs
-    as the single pass of N = L, a strided kernel as the first pass of
-    N = 16 L (radix 16 outermost, the boundary right after it)."""
+    compiled kernel of a length that is not a power of two: a whole-transform
+    kernel (rows = 1) runs as the single pass of N = L, a strided kernel as
+    the first pass of N = 16 L (radix 16 outermost, the boundary right after
+    it)."""
     out = []
     for i, k in enumerate(mofft.kernels()):
-        if k.precision_bits != bits or k.length not in SMALL_POWERS:
+        if (k.precision_bits != bits or k.rows != rows
+                or k.length & (k.length - 1) == 0):
             continue
         inner = list(reversed(k.stages[: k.stage_count]))
         if k.rows:

@@ -6,13 +6,15 @@ same bounds.  The oracle is FP64 direct summation (N <= 4096) or the
 compensated double-double FFT (larger N, and large batches), both pinned
 against each other and against closed forms in tests/test_oracle.py.
 """
+import json
 import math
+import os
 
 import numpy as np
 import pytest
 
 import oracle
-from conftest import seeded_complex, small_power_plans
+from conftest import seeded_complex, mixed_kernel_plans
 
 pytestmark = pytest.mark.gpu
 
@@ -136,11 +138,14 @@ def test_small_transform_tiles(gpu, n, bits, batch):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-def test_small_power_kernels(gpu, bits):
-    """Each kernel of a small-radix power length (3^k, 5^k, 6^k, 7^k, 10^k
-    up to 4096 points; chains of radix 3, 5, 6, 7, 9, 10 stages) against the
-    oracle, forward out of place and backward in place."""
-    plans = small_power_plans(gpu, bits)
+@pytest.mark.parametrize("rows", [1, 0])
+def test_mixed_kernels(gpu, bits, rows):
+    """Each kernel of a length that is not a power of two (every smooth
+    length up to 1024 points, the reference's mixed lengths, the powers of 3,
+    5, 6, 7 up to 3125) against the oracle, forward out of place and
+    backward in place: whole-transform kernels on N = L, strided kernels as
+    the first pass of N = 16 L."""
+    plans = mixed_kernel_plans(gpu, bits, rows)
     assert plans
     for i, n, radices, mask in plans:
         batch = max(1, min(8, 8192 // n))
@@ -177,3 +182,29 @@ def test_partial_output_groups(gpu, n, radices, mask, bits):
                            radices=radices, mask=mask)
         err = oracle.rel_l2(y, reference(xr, sign))
         assert err <= tol(n, bits), (n, desc.passes, direction, err)
+
+
+def _wisdom_cases():
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "mofft_amd", "wisdom", "gfx950.json")
+    with open(path) as f:
+        entries = json.load(f)["entries"]
+    return sorted({(e["length"], 32 if e["precision"] == "fp32" else 64)
+                   for e in entries})
+
+
+@pytest.mark.parametrize("n,bits", _wisdom_cases())
+def test_every_wisdom_plan(gpu, n, bits):
+    """Each tuned PRETUNED plan (the tuner times plans, it does not check
+    them) against the oracle, one transform forward."""
+    x = seeded_complex((1, n), seed=9500 + n)
+    xr = x.astype(np.complex64 if bits == 32 else np.complex128)
+    plan = gpu.Plan(n, 1, bits, gpu.FORWARD)
+    try:
+        assert plan.diagnostics.used_wisdom == 1
+        y = np.empty_like(xr)
+        plan.execute(xr, y)
+    finally:
+        plan.destroy()
+    err = oracle.rel_l2(y, reference(xr, -1))
+    assert err <= tol(n, bits), (n, bits, err)

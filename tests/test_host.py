@@ -16,7 +16,7 @@ from pathlib import Path
 import pytest
 
 import mofft_amd as mf
-from conftest import SMALL_POWERS, small_power_plans
+from conftest import SMALL_POWERS, mixed_kernel_plans
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -106,9 +106,10 @@ def test_no_device_means_unsupported_hardware():
 
 @pytest.mark.parametrize("n,bits,passes", [
     (1 << 25, 32, [512, 256, 256]), (1 << 25, 64, [512, 256, 256]),
-    (45, 32, [45]), (1000, 64, [1000]), (59049, 32, [243, 243]),
-    (117649, 64, [343, 343]), (100000, 32, [1000, 100]),
-    (161051, 64, [121, 121, 11]), (8748, 64, [243, 36]),
+    (45, 32, [45]), (24, 64, [24]), (1000, 64, [1000]),
+    (59049, 32, [243, 243]), (117649, 64, [343, 343]),
+    (100000, 32, [400, 250]), (33600, 32, [192, 175]),
+    (161051, 64, [121, 121, 11]), (8748, 64, [108, 81]),
 ])
 def test_model_schedule(n, bits, passes):
     """Lengths without a wisdom entry: the analytical planner's schedule
@@ -121,12 +122,18 @@ def test_model_schedule(n, bits, passes):
 
 
 @pytest.mark.parametrize("bits", [32, 64])
-def test_small_power_kernels_selected(bits):
-    """Every kernel of a small-radix power length is the one its plan runs
-    (whole-transform kernels as the single pass, strided kernels as the first
-    pass), so the GPU parity test over these plans covers each of them."""
-    plans = small_power_plans(mf, bits)
-    assert {mf.kernels()[i].length for i, *_ in plans} == set(SMALL_POWERS)
+@pytest.mark.parametrize("rows", [1, 0])
+def test_mixed_kernels_selected(bits, rows):
+    """Every kernel of a length that is not a power of two is the one its
+    plan runs (whole-transform kernels as the single pass, strided kernels as
+    the first pass), so the GPU parity test over these plans covers each of
+    them: every 2,3,5,7,11,13-smooth length up to 1024 and the larger powers."""
+    plans = mixed_kernel_plans(mf, bits, rows)
+    lengths = {mf.kernels()[i].length for i, *_ in plans}
+    smooth = [n for n in range(3, 1025) if n & (n - 1)
+              and max(p for p in range(2, n + 1) if n % p == 0
+                      and all(p % q for q in range(2, p))) <= 13]
+    assert set(smooth) <= lengths and set(SMALL_POWERS) <= lengths
     for i, n, radices, mask in plans:
         d = mf.describe(n, 1, bits, radices, mask)
         assert d.pass_kernel[0] == i, (n, radices, mask)
