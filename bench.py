@@ -7,7 +7,10 @@ the FP64 forward+backward in-place configuration the metric is quoted on.
 Prints ONE JSON line:
 
   value      GFLOP/s under the 5*N*log2(N) convention (tools/benchmark.cpp:709),
-             both directions counted, whole job, inputs resident in HBM
+             both directions counted, whole job, inputs resident in HBM: K
+             steps back to back between two HIP events
+  p50/p99    per-step latency from a run of K steps before that one, each
+             step between its own events
   roofline   dominant kernel vs the 8 TB/s HBM3E peak.  Algorithmic bytes are
              one read + one write of every point per transform direction
              (2*N*16 B per FP64 transform); a plan with P HBM passes is
@@ -189,22 +192,29 @@ def run_gpu(args):
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
+    # latency percentiles: K steps, each between its own events
     starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    for i in range(args.steps):
+        starts[i].record(stream)
+        step()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    # throughput: another K steps back to back between two events (an event
+    # between steps stops a step's first kernel from overlapping the
+    # previous step's last one)
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     wall0 = time.perf_counter()
     t_start.record(stream)
     for i in range(args.steps):
-        starts[i].record(stream)
         step()
-        ends[i].record(stream)
     t_end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
     total_ms = t_start.elapsed_time(t_end)
-    step_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
     ms = total_ms / args.steps
     flops = directions * 5.0 * n * math.log2(n) * batch
     gflops = flops / (ms * 1e-3) / 1e9

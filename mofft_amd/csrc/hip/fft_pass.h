@@ -170,12 +170,26 @@ __device__ __forceinline__ void st_stream(cx<T> *p, T a, T b) {
 #endif
 }
 
+/* (a.x + i a.y)(b.x + i b.y) with the fused multiply-adds written out, so
+ * every kernel rounds a complex product the same way whatever the compiler
+ * would contract in its context (a batch tile and a single-plan tile of the
+ * same chain stay bit-identical). */
+template <typename T>
+__device__ __forceinline__ cx<T> cprod(const cx<T> a, const cx<T> b) {
+  cx<T> r;
+  r.x = __builtin_fma(a.x, b.x, -(a.y * b.y));
+  r.y = __builtin_fma(a.x, b.y, a.y * b.x);
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ void cmul(T &re, T &im, const cx<T> w) {
-  const T r = re * w.x - im * w.y;
-  const T i = re * w.y + im * w.x;
-  re = r;
-  im = i;
+  cx<T> a;
+  a.x = re;
+  a.y = im;
+  const cx<T> r = cprod<T>(a, w);
+  re = r.x;
+  im = r.y;
 }
 
 template <typename T>
@@ -294,6 +308,36 @@ __device__ __forceinline__ bool tile_store(const mofft_pass_args &p) {
 constexpr int kMfma16 = -16;
 constexpr int radix_value(int r) { return r == kMfma16 ? 16 : r; }
 
+/* Sub-stage twiddle table of a kernel: entries OFF + (jj % NS) * (R - 1) + k
+ * for every sub-stage with NS > 1, in execution order.  When it is small
+ * next to the tile (at most a quarter of its LDS, and within the LDS cap)
+ * each tile copies it into LDS behind the tile area, loading it together
+ * with its data, so the sub-stages read their twiddles at LDS latency.
+ * Otherwise the sub-stages read it from global memory (L2), each issuing
+ * the next sub-stage's twiddle loads before the exchange that precedes it.
+ * The registry (mofft_compiler/registry.py, Entry.lds_bytes) sizes LDS by
+ * the same rule. */
+constexpr int kLdsCap = 150 * 1024;
+
+/* entries of the sub-stages R... when the first of them has stride ns */
+template <int NS0, int... R>
+constexpr int stage_tw_entries() {
+  constexpr int rs[] = {radix_value(R)...};
+  int ns = NS0, count = 0;
+  for (int i = 0; i < (int)sizeof...(R); ++i) {
+    if (ns > 1) count += ns * (rs[i] - 1);
+    ns *= rs[i];
+  }
+  return count;
+}
+
+template <typename T, int L, int TPT, int C, bool ROWS>
+constexpr bool stage_tw_in_lds(int count) {
+  const int tile = Tile<T, L, TPT, C, ROWS>::BYTES;
+  const int table = count * (int)sizeof(cx<T>);
+  return count > 0 && 4 * table <= tile && (tile + 15) / 16 * 16 + table <= kLdsCap;
+}
+
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
           int R, int... REST>
 struct Stage;
@@ -321,6 +365,30 @@ struct Stage {
   static constexpr int IT = (NB + TPT - 1) / TPT;
   static constexpr bool FULL = IT * TPT == NB;
   static constexpr bool LAST = sizeof...(REST) == 0;
+  /* the kernel's sub-stage table is in LDS (run_tile copies it there) */
+  static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
+      OFF + stage_tw_entries<NS, R, REST...>());
+  static constexpr bool PREFETCH_TW = true;
+
+  /* this sub-stage's twiddles w_{NS R}^(j k), k = 1..R-1, of every
+   * butterfly of the thread (j = jj % NS) */
+  static __device__ __forceinline__ void load_tw(const Ctx<T> &x,
+                                                 cx<T> (&w)[IT][R - 1]) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int jj = x.t + it * TPT;
+#pragma unroll
+      for (int k = 0; k < R - 1; ++k) {
+        w[it][k].x = T(1);
+        w[it][k].y = T(0);
+      }
+      if (FULL || jj < NB) {
+        const cx<T> *src = x.stw + OFF + (jj % NS) * (R - 1);
+#pragma unroll
+        for (int k = 0; k < R - 1; ++k) w[it][k] = src[k];
+      }
+    }
+  }
 
   /* first sub-stage: element e = jj + k*NB straight from HBM.  Every
    * load (data, then pass twiddles) is issued before any result is used:
@@ -489,8 +557,7 @@ struct Stage {
 #pragma unroll
             for (int k = 0; k < R; ++k) {
               const cx<T> a = h[it][k], b = l[it][k];
-              w[it][k].x = a.x * b.x - a.y * b.y;
-              w[it][k].y = a.x * b.y + a.y * b.x;
+              w[it][k] = cprod<T>(a, b);
             }
         }
       }
@@ -583,14 +650,30 @@ struct Stage {
   static __device__ __forceinline__ void run(const Ctx<T> &x,
                                              const mofft_pass_args &p,
                                              T (&vr)[IT][R], T (&vi)[IT][R]) {
+    if constexpr (NS > 1) {
+      cx<T> w[IT][R - 1];
+      load_tw(x, w);
+      run_tw(x, p, vr, vi, w);
+    } else {
+      cx<T> w[IT][1];
+      run_tw(x, p, vr, vi, w);
+    }
+  }
+
+  /* the same with this sub-stage's twiddles already loaded (w is unused
+   * when NS == 1) */
+  template <int W>
+  static __device__ __forceinline__ void run_tw(const Ctx<T> &x,
+                                                const mofft_pass_args &p,
+                                                T (&vr)[IT][R], T (&vi)[IT][R],
+                                                const cx<T> (&w)[IT][W]) {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       if constexpr (NS > 1) {
         const int jj = x.t + it * TPT;
         if (FULL || jj < NB) {
-          const cx<T> *w = x.stw + OFF + (jj % NS) * (R - 1);
 #pragma unroll
-          for (int k = 1; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], w[k - 1]);
+          for (int k = 1; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], w[it][k - 1]);
         }
       }
       Dft<R>::template run<T>(vr[it], vi[it]);
@@ -601,6 +684,15 @@ struct Stage {
                                      REST...>::type;
       T nr[Next::IT][Next::RAD];
       T ni[Next::IT][Next::RAD];
+      /* a global table: the next sub-stage's twiddle loads are in flight
+       * during the exchange instead of after it (FP32 N = 2^10..2^12 single
+       * pass 4-10 % faster).  Not in 1024-thread workgroups: a thread there
+       * has at most 128 VGPRs and the held twiddles spill (FP64 N = 2^14
+       * single pass 77 -> 127 us). */
+      constexpr bool PF = !TW_IN_LDS && Next::PREFETCH_TW && TPT * C < 1024;
+      using NW = cx<T>[Next::IT][PF ? Next::RAD - 1 : 1];
+      NW nw;
+      if constexpr (PF) Next::load_tw(x, nw);
 #if MOFFT_SPLIT_EXCHANGE
       put_lds(x, x.lds, vr);
       __syncthreads();
@@ -618,7 +710,8 @@ struct Stage {
       Next::load_lds(x, x.lds + TL::WORDS, ni);
       __syncthreads();
 #endif
-      Next::run(x, p, nr, ni);
+      if constexpr (PF) Next::run_tw(x, p, nr, ni, nw);
+      else Next::run(x, p, nr, ni);
     } else {
       if (ROWS || !tile_store<C>(p)) store_direct(x, vr, vi);
       else store_tile<T, L, TPT, C, ROWS, IT, R>(x, p, vr, vi);
@@ -666,34 +759,13 @@ __device__ __forceinline__ void setup_tile(Ctx<T> &x, const mofft_pass_args &p,
   x.tpass = (const cx<T> *)p.tw_pass + jr;
 }
 
-/* Sub-stage twiddle table of a kernel: entries OFF + (jj % NS) * (R - 1) + k
- * for every sub-stage with NS > 1, in execution order.  When it is small
- * next to the tile (at most a quarter of its LDS, and within the LDS cap)
- * each tile copies it into LDS behind the tile area, loading it together
- * with its data, so the sub-stages read their twiddles at LDS latency
- * instead of waiting for a global load after every exchange.  The registry
- * (mofft_compiler/registry.py, Entry.lds_bytes) sizes LDS by the same rule. */
-constexpr int kLdsCap = 150 * 1024;
-
-template <int... R>
-constexpr int stage_tw_entries() {
-  constexpr int rs[] = {radix_value(R)...};
-  int ns = 1, count = 0;
-  for (int i = 0; i < (int)sizeof...(R); ++i) {
-    if (ns > 1) count += ns * (rs[i] - 1);
-    ns *= rs[i];
-  }
-  return count;
-}
-
 template <typename T, int L, int TPT, int C, bool ROWS, int... R>
 struct StageTw {
-  static constexpr int COUNT = stage_tw_entries<R...>();
+  static constexpr int COUNT = stage_tw_entries<1, R...>();
   static constexpr int TILE = Tile<T, L, TPT, C, ROWS>::BYTES;
   static constexpr int OFFSET = (TILE + 15) / 16 * 16;
   static constexpr int TABLE = COUNT * (int)sizeof(cx<T>);
-  static constexpr bool IN_LDS =
-      COUNT > 0 && 4 * TABLE <= TILE && OFFSET + TABLE <= kLdsCap;
+  static constexpr bool IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(COUNT);
   static constexpr int BYTES = IN_LDS ? OFFSET + TABLE : TILE;
 };
 
@@ -703,6 +775,8 @@ __device__ __forceinline__ void run_tile(const Ctx<T> &x,
                                          const mofft_pass_args &p) {
   using First = typename StageSel<T, L, TPT, C, ROWS, 1, 0, R0, R...>::type;
   using SW = StageTw<T, L, TPT, C, ROWS, R0, R...>;
+  static_assert(SW::COUNT == 0 || First::TW_IN_LDS == SW::IN_LDS,
+                "one LDS rule for the sub-stage table");
   T vr[First::IT][First::RAD];
   T vi[First::IT][First::RAD];
   if constexpr (SW::IN_LDS) {
@@ -819,6 +893,9 @@ struct MStage {
   static constexpr int GROUPS = (C * NB) / 32;
   static constexpr int IT = GROUPS / (WAVES > 0 ? WAVES : 1);
   static constexpr bool LAST = sizeof...(REST) == 0;
+  static constexpr bool PREFETCH_TW = false;
+  static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
+      OFF + stage_tw_entries<NS, 16, REST...>());
   static_assert(NT % 64 == 0 && (C * NB) % 32 == 0 && GROUPS % WAVES == 0,
                 "matrix-core tile shape");
   /* the VALU radix-16 stage whose thread layout HBM loads / stores use */
