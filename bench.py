@@ -457,11 +457,11 @@ def run_sweep(args):
         for form in ("plain", "mfma"):
             p = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False, mfma=form == "mfma")
             q = mf.Plan(n, batch, 32, mf.BACKWARD, in_place=False, mfma=form == "mfma")
-            for _ in range(3):
+            for _ in range(10):
                 p.execute_async(x, y, sh)
             torch.cuda.synchronize()
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            reps = 20
+            reps = 30
             e0.record()
             for _ in range(reps):
                 p.execute_async(x, y, sh)
