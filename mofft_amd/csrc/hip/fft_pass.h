@@ -174,11 +174,18 @@ __device__ __forceinline__ void st_stream(cx<T> *p, T a, T b) {
  * every kernel rounds a complex product the same way whatever the compiler
  * would contract in its context (a batch tile and a single-plan tile of the
  * same chain stay bit-identical). */
+__device__ __forceinline__ float fused(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+__device__ __forceinline__ double fused(double a, double b, double c) {
+  return __builtin_fma(a, b, c);
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> cprod(const cx<T> a, const cx<T> b) {
   cx<T> r;
-  r.x = __builtin_fma(a.x, b.x, -(a.y * b.y));
-  r.y = __builtin_fma(a.x, b.y, a.y * b.x);
+  r.x = fused(a.x, b.x, -(a.y * b.y));
+  r.y = fused(a.x, b.y, a.y * b.x);
   return r;
 }
 
