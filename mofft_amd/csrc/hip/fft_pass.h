@@ -532,7 +532,9 @@ struct Stage {
           w[it][k].x = T(1);
           w[it][k].y = T(0);
         }
-      if (x.pass_tw) {
+      /* whole-transform (row) tiles are single-pass plans: no pass
+       * twiddles, and no registers held for them */
+      if (!ROWS && x.pass_tw) {
         if (x.pass_table) {
 #pragma unroll
           for (int it = 0; it < IT; ++it) {
@@ -575,7 +577,7 @@ struct Stage {
           vr[it][k] = x.swap_in ? raw[it][k].y : raw[it][k].x;
           vi[it][k] = x.swap_in ? raw[it][k].x : raw[it][k].y;
         }
-      if (x.pass_tw) {
+      if (!ROWS && x.pass_tw) {
 #pragma unroll
         for (int it = 0; it < IT; ++it)
 #pragma unroll
@@ -843,6 +845,8 @@ struct PassKernel {
       configured = true;
     }
     if (!a) return 0; /* configure only (e.g. before a stream capture) */
+    /* row tiles never carry pass twiddles (see Stage::load_first) */
+    if (ROWS && (a->flags & MOFFT_PASS_TWIDDLE)) return (int)hipErrorInvalidValue;
     const int64_t blocks = (a->columns + C - 1) / C;
     if (blocks <= 0) return 0;
     if (blocks > 0x7fffffff) return (int)hipErrorInvalidValue;
