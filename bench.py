@@ -63,6 +63,8 @@ def parse():
     ap.add_argument("--owned-tiles", type=int, default=0)
     ap.add_argument("--batch-bits", type=int, default=32, choices=[32, 64])
     ap.add_argument("--item-tiles", type=int, default=0)
+    ap.add_argument("--batch-grid", type=int, default=0,
+                    help="c4: persistent grid size (0: resident workgroups per CU x CUs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweep-max", type=int, default=20)
@@ -316,7 +318,7 @@ def run_batch(args):
     ins = [torch.view_as_complex(torch.randn(n, 2, device=dev, generator=g, dtype=fdt))
            for n in lengths]
     outs = [a if ip else torch.empty_like(a) for a, ip in zip(ins, inplace)]
-    bp = mf.BatchPlan(lengths, dirs, ins, outs, bits, graph=True,
+    bp = mf.BatchPlan(lengths, dirs, ins, outs, bits, graph=True, grid=args.batch_grid,
                       owned_tiles=args.owned_tiles, item_tiles=args.item_tiles)
     d = bp.description()
 

@@ -15,7 +15,7 @@
  *   - a multi-pass transform gets one segment per pass; in-place and
  *     three-or-more-pass transforms get a private slice of the workspace;
  *   - a multi-pass transform of at most twice the fair share of tiles per
- *     resident workgroup is OWNED (one queue item, all passes run by one
+ *     workgroup of the grid is OWNED (one queue item, all passes run by one
  *     workgroup); a larger one is SHARED (items of a few tiles, later passes
  *     wait for earlier ones through counters);
  *   - queue order: shared first passes, owned transforms longest first,
@@ -302,13 +302,13 @@ static int64_t seg_items(const mofft_batch_segment *sg) {
  * transforms (longest first), gathered row segments, shared later passes
  * level by level; then the continuation passes of owned transforms. */
 static mofft_status build_queue(mofft_batch_plan *p, const size_t *order,
-                                int64_t resident) {
+                                int64_t workers) {
   size_t cap = 0;
   mofft_status st = MOFFT_SUCCESS;
   size_t *last = (size_t *)malloc(p->count * sizeof(size_t));
   p->shared = (unsigned char *)calloc(p->count, 1);
   if (!last || !p->shared) { free(last); return MOFFT_ALLOCATION_FAILURE; }
-  /* fair share of tiles per resident workgroup */
+  /* fair share of tiles per workgroup of the grid */
   int64_t all = 0;
   for (size_t i = 0; i < p->bl_count; ++i) {
     const batch_length *b = &p->bl[i];
@@ -321,7 +321,7 @@ static mofft_status build_queue(mofft_batch_plan *p, const size_t *order,
     const mofft_schedule *s = &p->bl[p->length_index[t]].s;
     if (s->pass_count > 1) all += transform_tiles(s);
   }
-  const int64_t share = resident > 0 ? all / resident : 0;
+  const int64_t share = workers > 0 ? all / workers : 0;
   /* Owned up to twice a fair share; shared passes in items of an eighth
    * of a share, so each item's cross-workgroup synchronisation (an L2
    * write-back and invalidate on gfx950) is amortised over several tiles.
@@ -422,7 +422,7 @@ static mofft_status build_queue(mofft_batch_plan *p, const size_t *order,
     }
   }
   if (st == MOFFT_SUCCESS &&
-      (items > (int64_t)INT32_MAX - resident || p->seg_count > (size_t)INT32_MAX))
+      (items > (int64_t)INT32_MAX - workers || p->seg_count > (size_t)INT32_MAX))
     st = MOFFT_UNSUPPORTED_SIZE;
   p->total_items = items;
   free(last);
@@ -545,8 +545,8 @@ mofft_status mofft_batch_plan_create(mofft_batch_plan **plan,
     st = MOFFT_ALLOCATION_FAILURE;
     goto fail;
   }
-  /* launch shape: every resident 256-thread slot of the kernel with the
-   * largest LDS tile among the kinds this batch uses */
+  /* launch shape: 256-thread workgroups, at most as many per CU as fit
+   * with the largest LDS tile among the kinds this batch uses */
   for (size_t i = 0; i < p->bl_count; ++i)
     for (int k = 0; k < p->bl[i].s.pass_count; ++k) {
       const mofft_registry_entry *e = kind_of(&p->bl[i].s, k);
@@ -559,8 +559,13 @@ mofft_status mofft_batch_plan_create(mofft_batch_plan **plan,
     goto fail;
   }
   p->blocks_per_cu = bpc;
-  const int64_t resident = (int64_t)bpc * cus;
-  st = build_queue(p, order, resident);
+  /* two workgroups per CU unless fewer fit: measured on 4096 FP32
+   * transforms of N = 2^6..2^18 (MI355X, 4 resident per CU), grids of
+   * 1024 / 768 / 512 / 384 workgroups ran 1.55 / 1.40 / 1.30 / 1.43 ms;
+   * FP64 (2 resident per CU) is flat from 512 to 640 */
+  int64_t workers = (int64_t)(bpc < 2 ? bpc : 2) * cus;
+  if (p->options.grid) workers = (int64_t)p->options.grid;
+  st = build_queue(p, order, workers);
   if (st != MOFFT_SUCCESS) goto fail;
   if (mofft_hip_malloc(&p->d_segs, p->seg_count * sizeof(mofft_batch_segment)) != 0 ||
       mofft_hip_malloc(&p->d_counters, (2 + p->queue_count) * sizeof(unsigned int)) != 0) {
@@ -569,7 +574,7 @@ mofft_status mofft_batch_plan_create(mofft_batch_plan **plan,
   }
   st = bind_buffers(p, inputs, outputs);
   if (st != MOFFT_SUCCESS) goto fail;
-  int64_t grid = p->options.grid ? (int64_t)p->options.grid : resident;
+  int64_t grid = workers;
   if (grid > p->total_items) grid = p->total_items;
   if (grid < 1) grid = 1;
   p->grid = (int)grid;
