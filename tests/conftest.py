@@ -101,3 +101,21 @@ def mixed_kernel_plans(mofft, bits, rows):
         else:
             out.append((i, 16 * k.length, [16] + inner, 1))
     return out
+
+
+def random_smooth_lengths(seed, count, limit):
+    """`count` distinct lengths in [2, limit] whose prime factors are radices
+    (2, 3, 5, 7, 11, 13), drawn log-uniformly from a seed, sorted."""
+    rng = np.random.default_rng(seed)
+    out = set()
+    while len(out) < count:
+        n = 1
+        target = np.exp(rng.uniform(np.log(2), np.log(limit)))
+        while n * 2 <= target:
+            q = int(rng.choice([2, 2, 2, 3, 3, 5, 5, 7, 11, 13]))
+            if n * q > limit:
+                break
+            n *= q
+        if n >= 2:
+            out.add(n)
+    return sorted(out)

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import seeded_complex, mixed_kernel_plans
+from conftest import seeded_complex, mixed_kernel_plans, random_smooth_lengths
 
 pytestmark = pytest.mark.gpu
 
@@ -208,3 +208,19 @@ def test_every_wisdom_plan(gpu, n, bits):
         plan.destroy()
     err = oracle.rel_l2(y, reference(xr, -1))
     assert err <= tol(n, bits), (n, bits, err)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_random_smooth_lengths(gpu, bits):
+    """40 seeded 13-smooth lengths up to 2^20 on their PRETUNED plans (one to
+    three passes of any mix of radices) against the oracle, forward out of
+    place and backward in place, batches of 1 to 3."""
+    for n in random_smooth_lengths(500 + bits, 40, 1 << 20):
+        batch = 1 + n % 3
+        x = seeded_complex((batch, n), seed=9700 + n)
+        xr = x.astype(np.complex64 if bits == 32 else np.complex128)
+        for direction, sign, in_place in ((gpu.FORWARD, -1, False),
+                                          (gpu.BACKWARD, 1, True)):
+            y, desc = run_host(gpu, n, batch, bits, direction, in_place, x)
+            err = oracle.rel_l2(y, reference(xr, sign))
+            assert err <= tol(n, bits), (n, desc.passes, direction, err)

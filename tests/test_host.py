@@ -16,7 +16,7 @@ from pathlib import Path
 import pytest
 
 import mofft_amd as mf
-from conftest import SMALL_POWERS, mixed_kernel_plans
+from conftest import SMALL_POWERS, mixed_kernel_plans, random_smooth_lengths
 
 ROOT = Path(__file__).resolve().parent.parent
 
@@ -138,6 +138,25 @@ def test_mixed_kernels_selected(bits, rows):
         d = mf.describe(n, 1, bits, radices, mask)
         assert d.pass_kernel[0] == i, (n, radices, mask)
         assert d.passes == ([n] if mask == 0 else [n // 16, 16])
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_random_smooth_length_schedules(bits):
+    """300 seeded 13-smooth lengths up to 2^26: every one has a schedule
+    whose passes and radix chain multiply to N, whose pass boundaries are
+    the mask bits, with every pass a compiled kernel of that length; up to
+    1024 points it is one pass (every smooth length there has a row
+    kernel)."""
+    ks = mf.kernels()
+    for n in random_smooth_lengths(77 + bits, 300, 1 << 26):
+        d = mf.describe(n, 1, bits)
+        assert math.prod(d.passes) == n and math.prod(d.chain) == n, n
+        assert bin(d.pass_mask).count("1") == d.pass_count - 1, n
+        for i in range(d.pass_count):
+            k = ks[d.pass_kernel[i]]
+            assert k.precision_bits == bits and k.length == d.passes[i], (n, i)
+        if n <= 1024:
+            assert d.pass_count == 1, n
 
 
 @pytest.mark.parametrize("bits", [32, 64])
