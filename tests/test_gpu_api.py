@@ -246,3 +246,38 @@ def test_capture_before_first_launch(gpu, torch_dev):
         torch.cuda.synchronize()
     assert torch.equal(got, out)
     assert oracle.rel_l2(got.cpu().numpy(), oracle.fft_dd(x, -1)) <= tol(n, 32)
+
+
+@pytest.mark.parametrize("n,bits", [(1 << 28, 32), (1 << 26, 64), (3 ** 17, 32)])
+def test_largest_lengths(gpu, torch_dev, n, bits):
+    """Lengths of 2^26..2^28 points (1-2 GiB per transform, three passes)
+    through size-independent checks on the device: an impulse at n0 gives
+    X[k] = w_N^(k n0) (closed form, FP64 on the device), and forward out
+    of place then backward in place returns N x."""
+    torch = torch_dev
+    cdt = torch.complex64 if bits == 32 else torch.complex128
+    s = torch.cuda.current_stream().cuda_stream
+    n0 = 12345
+    x = torch.zeros(n, dtype=cdt, device="cuda")
+    x[n0] = 1.0
+    y = torch.empty_like(x)
+    with gpu.Plan(n, 1, bits, gpu.FORWARD) as p:
+        assert p.description().pass_count == 3
+        p.execute_async(x, y, s)
+        torch.cuda.synchronize()
+    k = torch.arange(n, dtype=torch.int64, device="cuda")
+    ang = (k * n0 % n).to(torch.float64) * (-2.0 * math.pi / n)
+    num = ((y.real.double() - torch.cos(ang)) ** 2 + (y.imag.double() - torch.sin(ang)) ** 2).sum()
+    err = math.sqrt(num.item() / n)
+    del k, ang
+    assert err <= tol(n, bits), ("impulse", n, bits, err)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n)
+    x = torch.randn(n, dtype=cdt, device="cuda", generator=g)
+    with gpu.Plan(n, 1, bits, gpu.FORWARD) as f, \
+            gpu.Plan(n, 1, bits, gpu.BACKWARD, in_place=True) as b:
+        f.execute_async(x, y, s)
+        b.execute_async(y, y, s)
+        torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x)).item()
+    assert rt <= 2 * tol(n, bits), ("round trip", n, bits, rt)
