@@ -248,12 +248,13 @@ def test_capture_before_first_launch(gpu, torch_dev):
     assert oracle.rel_l2(got.cpu().numpy(), oracle.fft_dd(x, -1)) <= tol(n, 32)
 
 
-@pytest.mark.parametrize("n,bits", [(1 << 28, 32), (1 << 26, 64), (3 ** 17, 32)])
+@pytest.mark.parametrize("n,bits", [(1 << 30, 32), (1 << 28, 32), (1 << 26, 64),
+                                    (3 ** 17, 32)])
 def test_largest_lengths(gpu, torch_dev, n, bits):
-    """Lengths of 2^26..2^28 points (1-2 GiB per transform, three passes)
-    through size-independent checks on the device: an impulse at n0 gives
-    X[k] = w_N^(k n0) (closed form, FP64 on the device), and forward out
-    of place then backward in place returns N x."""
+    """Lengths of 2^26 points up to the largest one (2^30, 8 GiB per FP32
+    transform; three passes) through size-independent checks on the device:
+    an impulse at n0 gives X[k] = w_N^(k n0) (closed form, FP64 on the
+    device), and forward out of place then backward in place returns N x."""
     torch = torch_dev
     cdt = torch.complex64 if bits == 32 else torch.complex128
     s = torch.cuda.current_stream().cuda_stream
@@ -265,11 +266,15 @@ def test_largest_lengths(gpu, torch_dev, n, bits):
         assert p.description().pass_count == 3
         p.execute_async(x, y, s)
         torch.cuda.synchronize()
-    k = torch.arange(n, dtype=torch.int64, device="cuda")
-    ang = (k * n0 % n).to(torch.float64) * (-2.0 * math.pi / n)
-    num = ((y.real.double() - torch.cos(ang)) ** 2 + (y.imag.double() - torch.sin(ang)) ** 2).sum()
-    err = math.sqrt(num.item() / n)
-    del k, ang
+    num = 0.0
+    for lo in range(0, n, 1 << 26):
+        hi = min(n, lo + (1 << 26))
+        k = torch.arange(lo, hi, dtype=torch.int64, device="cuda")
+        ang = (k * n0 % n).to(torch.float64) * (-2.0 * math.pi / n)
+        ys = y[lo:hi]
+        num += ((ys.real.double() - torch.cos(ang)) ** 2
+                + (ys.imag.double() - torch.sin(ang)) ** 2).sum().item()
+    err = math.sqrt(num / n)
     assert err <= tol(n, bits), ("impulse", n, bits, err)
     g = torch.Generator(device="cuda")
     g.manual_seed(n)
