@@ -70,22 +70,27 @@ static size_t chunk_bytes(void) {
   return cached;
 }
 
-/* Largest full pass-twiddle table (bytes); MOFFT_PASS_TABLE_BYTES in the
- * environment overrides it (a tuning knob; 0 = always the two-level
- * table; read once per process). */
-static size_t pass_table_bytes(void) {
+/* Largest full pass-twiddle table (bytes) of a precision;
+ * MOFFT_PASS_TABLE_BYTES in the environment overrides both (a tuning knob;
+ * 0 = always the two-level table; read once per process). */
+static size_t pass_table_bytes(int precision_bits) {
   static int cached = 0;
-  static size_t v = MOFFT_PASS_TABLE_BYTES;
+  static int overridden = 0;
+  static size_t v = 0;
   if (!cached) {
     const char *env = getenv("MOFFT_PASS_TABLE_BYTES");
     if (env && *env) {
       char *end = NULL;
       const unsigned long long x = strtoull(env, &end, 10);
-      if (end && *end == '\0') v = (size_t)x;
+      if (end && *end == '\0') {
+        v = (size_t)x;
+        overridden = 1;
+      }
     }
     cached = 1;
   }
-  return v;
+  if (overridden) return v;
+  return precision_bits == 64 ? MOFFT_PASS_TABLE_BYTES_F64 : MOFFT_PASS_TABLE_BYTES;
 }
 
 /* src/mofft.c:27-28 - the radix set the API accepts. */
@@ -329,7 +334,7 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   for (int i = 1; i < s->pass_count; ++i) {
     mofft_pass *p = &s->passes[i];
     const size_t entries = p->span * p->length;
-    if (sat_mul(entries, cb0) <= pass_table_bytes()) {
+    if (sat_mul(entries, cb0) <= pass_table_bytes(precision_bits)) {
       p->pass_table = 1;
       p->pass_offset = count;
       count += entries;

@@ -38,8 +38,12 @@ typedef struct {
  * bytes per pass the factors come from a full table laid out so adjacent
  * columns read adjacent entries (one coalesced load per point, L2/Infinity
  * Cache resident); beyond it from a two-level table (two gathered loads and
- * a complex product per point). */
+ * a complex product per point).  Measured on MI355X (2^24 points per
+ * launch): FP32 N = 2^16..2^20 run 3-7 % faster on full tables up to 8 MiB;
+ * FP64 N = 2^19 runs 3.6 % slower on its 8 MiB table than on the two-level
+ * one, so FP64 keeps full tables to 4 MiB. */
 #define MOFFT_PASS_TABLE_BYTES ((size_t)8 << 20)
+#define MOFFT_PASS_TABLE_BYTES_F64 ((size_t)4 << 20)
 
 typedef struct {
   const mofft_kernel_set *ks;          /* kernel indices refer to this set */
