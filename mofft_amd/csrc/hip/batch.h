@@ -244,17 +244,17 @@ __global__ void batch_reset_kernel(unsigned int *counters, int64_t count) {
  * file that defines K). */
 template <void (*K)(mofft_batch_header)>
 struct BatchLauncher {
+  /* The LDS opt-in covers the largest tile any batch may use, so a later
+   * batch with a larger one needs no second call on that device. */
+  static int allow(int lds_bytes) {
+    static std::atomic<unsigned long long> configured{0};
+    if (lds_bytes > kLdsCap) return (int)hipErrorInvalidValue;
+    if (lds_bytes <= 65536) return 0;
+    return allow_lds(configured, reinterpret_cast<const void *>(K), kLdsCap);
+  }
   static int occupancy(int lds_bytes, int *blocks_per_cu, int *cu_count) {
-    static bool configured = false;
-    if (!configured) {
-      if (lds_bytes > 65536) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(K),
-            hipFuncAttributeMaxDynamicSharedMemorySize, lds_bytes);
-        if (e != hipSuccess) return (int)e;
-      }
-      configured = true;
-    }
+    const int rc = allow(lds_bytes);
+    if (rc != 0) return rc;
     int n = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
         &n, reinterpret_cast<const void *>(K), kBatchThreads, lds_bytes);
@@ -271,6 +271,8 @@ struct BatchLauncher {
   }
   static int launch(const mofft_batch_header *h, int grid, int lds_bytes,
                     hipStream_t s) {
+    const int rc = allow(lds_bytes);
+    if (rc != 0) return rc;
     const int64_t words = 2 + h->counter_count;
     const unsigned rb = (unsigned)((words + 255) / 256);
     hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(rb), dim3(256), 0, s,

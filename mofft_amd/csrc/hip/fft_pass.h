@@ -35,6 +35,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "../mofft_abi.h"
 #include "mofft_codelets.h"
 
@@ -826,6 +828,22 @@ fft_pass_kernel(mofft_pass_args p) {
   run_tile<T, L, TPT, C, ROWS, R0, R...>(x, p);
 }
 
+/* A launch with more than 64 KiB of dynamic LDS needs the kernel's
+ * opt-in, once per device (`done` holds one bit per device ordinal). */
+inline int allow_lds(std::atomic<unsigned long long> &done, const void *fn,
+                     int bytes) {
+  if (bytes <= 65536) return 0;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  const unsigned long long bit = dev < 64 ? 1ull << dev : 0ull;
+  if (bit && (done.load(std::memory_order_acquire) & bit)) return 0;
+  e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return (int)e;
+  done.fetch_or(bit, std::memory_order_acq_rel);
+  return 0;
+}
+
 template <typename T, int L, int TPT, int C, bool ROWS, int... R>
 struct PassKernel {
   static constexpr int THREADS = TPT * C;
@@ -833,17 +851,12 @@ struct PassKernel {
   static constexpr bool HAS_MFMA = ((R == kMfma16) || ...);
   static constexpr int LDS = StageTw<T, L, TPT, C, ROWS, R...>::BYTES;
   static int launch(const mofft_pass_args *a, hipStream_t s) {
-    static bool configured = false;
-    if (!configured) {
-      if (LDS > 65536) {
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void *>(
-                &fft_pass_kernel<T, L, TPT, C, ROWS, R...>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, LDS);
-        if (e != hipSuccess) return (int)e;
-      }
-      configured = true;
-    }
+    static std::atomic<unsigned long long> configured{0};
+    const int rc = allow_lds(
+        configured,
+        reinterpret_cast<const void *>(&fft_pass_kernel<T, L, TPT, C, ROWS, R...>),
+        LDS);
+    if (rc != 0) return rc;
     if (!a) return 0; /* configure only (e.g. before a stream capture) */
     /* row tiles never carry pass twiddles (see Stage::load_first) */
     if (ROWS && (a->flags & MOFFT_PASS_TWIDDLE)) return (int)hipErrorInvalidValue;
