@@ -286,3 +286,35 @@ def test_largest_lengths(gpu, torch_dev, n, bits):
         torch.cuda.synchronize()
     rt = (torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x)).item()
     assert rt <= 2 * tol(n, bits), ("round trip", n, bits, rt)
+
+
+@pytest.mark.parametrize("n,batch,bits", [(20736, 1000, 64), (1 << 21, 20, 32),
+                                          (1 << 16, 300, 64), (98415, 200, 32)])
+def test_multi_chunk_batches(gpu, torch_dev, n, batch, bits):
+    """Batches larger than one 128 MiB chunk (2-3 chunks, the last one
+    partial; two- and three-pass plans): forward in place, then a sample
+    of transforms (first, last, the first of the last chunk) against the
+    oracle, then backward in place back to N x for the whole batch."""
+    torch = torch_dev
+    cdt = torch.complex64 if bits == 32 else torch.complex128
+    s = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device="cuda")
+    g.manual_seed(n + batch)
+    x = torch.randn(batch, n, dtype=cdt, device="cuda", generator=g)
+    y = x.clone()
+    with gpu.Plan(n, batch, bits, gpu.FORWARD, in_place=True) as f, \
+            gpu.Plan(n, batch, bits, gpu.BACKWARD, in_place=True) as b:
+        d = f.description()
+        chunks = -(-batch // d.chunk_transforms)
+        assert chunks >= 2 and batch % d.chunk_transforms != 0
+        f.execute_async(y, y, s)
+        torch.cuda.synchronize()
+        last_chunk = (chunks - 1) * d.chunk_transforms
+        for t in (0, last_chunk - 1, last_chunk, batch - 1):
+            ref = oracle.fft_dd(x[t:t + 1].cpu().numpy().astype(np.complex128), -1)
+            err = oracle.rel_l2(y[t:t + 1].cpu().numpy(), ref)
+            assert err <= tol(n, bits), ("forward", t, err)
+        b.execute_async(y, y, s)
+        torch.cuda.synchronize()
+    rt = (torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x)).item()
+    assert rt <= 2 * tol(n, bits), ("round trip", rt)
