@@ -39,14 +39,16 @@ MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 # instead of one pass per radix stage
 SMALL_POWERS = (25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729,
                 1000, 1296, 2187, 2401, 3125)
-SMOOTH_MAX = 1024
+SMOOTH_MAX = 1024          # single-pass and strided kernels
+SMOOTH_ROWS_MAX = 4096     # single-pass kernels only (a pass of a longer
+                           # transform is at most 1024 points here)
 
 
 def smooth_lengths(limit: int = SMOOTH_MAX) -> tuple[int, ...]:
     """Every length up to `limit` whose prime factors are radices (2, 3, 5,
-    7, 11, 13): each gets a single-pass and a strided kernel, so a smooth
-    transform of up to 1024 points is one HBM pass and one of up to about a
-    million points two."""
+    7, 11, 13).  Those up to 1024 get a single-pass and a strided kernel,
+    those up to 4096 a single-pass one: a smooth transform of up to 4096
+    points is one HBM pass and one of up to about a million points two."""
     out = []
     for n in range(2, limit + 1):
         m = n
@@ -269,7 +271,7 @@ def entries() -> list[Entry]:
             add(_make(precision, length, 1, _pow2_chain(length)))
         for r in STAGE_RADICES:
             add(_make(precision, r, 1, (r,)))
-        for length in MIXED_ROWS + SMALL_POWERS + smooth_lengths():
+        for length in MIXED_ROWS + SMALL_POWERS + smooth_lengths(SMOOTH_ROWS_MAX):
             add(_make(precision, length, 1, chain_for(length)))
         for k in range(7, rows_max.bit_length()):
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))

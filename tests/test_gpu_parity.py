@@ -2,7 +2,7 @@
 
 Bars (north star): plain FP32 relative L2 error <= 1e-6*log2(N), FP64
 <= 1e-15*log2(N), and forward->backward round trips back to N*x within the
-same bounds.  The oracle is FP64 direct summation (N <= 4096) or the
+same bounds.  The oracle is FP64 direct summation (N <= 1024) or the
 compensated double-double FFT (larger N, and large batches), both pinned
 against each other and against closed forms in tests/test_oracle.py.
 """
@@ -25,7 +25,7 @@ def tol(n, precision_bits):
 
 def reference(x, sign):
     n = x.shape[-1]
-    if n <= 4096 and x.size <= 4 * 4096:
+    if n <= 1024 and x.size <= 4 * 1024:
         return oracle.dft(x, sign)
     return oracle.fft_dd(x, sign)
 

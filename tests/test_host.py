@@ -145,7 +145,7 @@ def test_random_smooth_length_schedules(bits):
     """300 seeded 13-smooth lengths up to 2^26: every one has a schedule
     whose passes and radix chain multiply to N, whose pass boundaries are
     the mask bits, with every pass a compiled kernel of that length; up to
-    1024 points it is one pass (every smooth length there has a row
+    4096 points it is one pass (every smooth length there has a row
     kernel)."""
     ks = mf.kernels()
     for n in random_smooth_lengths(77 + bits, 300, 1 << 26):
@@ -155,7 +155,7 @@ def test_random_smooth_length_schedules(bits):
         for i in range(d.pass_count):
             k = ks[d.pass_kernel[i]]
             assert k.precision_bits == bits and k.length == d.passes[i], (n, i)
-        if n <= 1024:
+        if n <= 4096:
             assert d.pass_count == 1, n
 
 
