@@ -382,13 +382,32 @@ static size_t schedule_workspace(const mofft_schedule *s,
 
 typedef struct {
   int count;
+  int weight; /* sum of pass_weight, tenths of a pass */
   size_t lengths[MOFFT_MAX_PLAN_DEPTH];
   int kernels[MOFFT_MAX_PLAN_DEPTH];
 } pass_split;
 
-/* better = fewer passes, then larger smallest pass, then larger first pass */
+/* Cost of a strided pass in units of one read + write of the batch at the
+ * speed of the short-column kernels, from MI355X pass timings: L <= 512
+ * runs at 5.0-5.8 TB/s (1); L = 1024 at 3.6-4.0 TB/s over 128 MiB (one
+ * 1024-thread workgroup per CU) but close to the others on a single
+ * multi-GiB transform (2^30: three 1024-point passes 4 % faster than four
+ * shorter ones), hence 1.3; L = 2048 three times slower than L = 256 (3);
+ * runs narrower than 128 bytes (C adjacent columns) cost at least 2.5
+ * (FP32 3 x 2^20 as 4096 (64-byte runs) x 768: 305 us, as three passes of
+ * wide runs 132 us). */
+static int pass_weight(const mofft_registry_entry *e) {
+  const int cb = e->precision_bits == 32 ? 8 : 16;
+  int w = e->length <= 512 ? 10 : e->length <= 1024 ? 13 : 30;
+  if (e->columns_per_block * cb < 128 && w < 25) w = 25;
+  return w; /* tenths: integer sums compare exactly */
+}
+
+/* better = lower HBM cost, then fewer passes, then larger smallest pass,
+ * then larger first pass */
 static int split_better(const pass_split *a, const pass_split *b) {
   if (b->count == 0) return 1;
+  if (a->weight != b->weight) return a->weight < b->weight;
   if (a->count != b->count) return a->count < b->count;
   size_t amin = SIZE_MAX, bmin = SIZE_MAX;
   for (int i = 0; i < a->count; ++i) if (a->lengths[i] < amin) amin = a->lengths[i];
@@ -407,7 +426,8 @@ static void search_splits(const mofft_kernel_set *ks, int precision_bits,
     return;
   }
   if (cur->count >= max_passes) return;
-  if (best->count != 0 && cur->count + 1 > best->count) return;
+  /* every further pass costs at least 1 */
+  if (best->count != 0 && cur->weight + 10 > best->weight) return;
   for (int i = 0; i < *ks->count; ++i) {
     const mofft_registry_entry *e = &ks->entries[i];
     if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
@@ -421,10 +441,15 @@ static void search_splits(const mofft_kernel_set *ks, int precision_bits,
           f->length == e->length) { dup = 1; break; }
     }
     if (dup) continue;
+    /* the weights are pass-kernel timings; the persistent batch's
+     * 256-thread kinds were measured as passes of equal cost */
+    const int w = ks == PASS_KERNELS ? pass_weight(e) : 10;
     cur->lengths[cur->count] = l;
     cur->kernels[cur->count] = i;
     cur->count++;
+    cur->weight += w;
     search_splits(ks, precision_bits, remaining / l, cur, best, max_passes);
+    cur->weight -= w;
     cur->count--;
   }
 }
@@ -447,8 +472,12 @@ static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
   pass_split cur, best;
   memset(&cur, 0, sizeof(cur));
   memset(&best, 0, sizeof(best));
-  for (int max_passes = 2; max_passes <= 12 && best.count == 0; max_passes += 2)
+  /* a plan of more than max_passes passes costs more than max_passes, so
+   * the search stops once the best plan found costs no more than that */
+  for (int max_passes = 2; max_passes <= 12; max_passes += 2) {
     search_splits(ks, precision_bits, length, &cur, &best, max_passes);
+    if (best.count != 0 && best.weight <= 10 * max_passes) break;
+  }
   if (best.count == 0) return MOFFT_UNSUPPORTED_SIZE;
   /* concatenate: the last pass is outermost */
   size_t d = 0;

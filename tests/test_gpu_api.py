@@ -252,7 +252,8 @@ def test_capture_before_first_launch(gpu, torch_dev):
                                     (3 ** 17, 32)])
 def test_largest_lengths(gpu, torch_dev, n, bits):
     """Lengths of 2^26 points up to the largest one (2^30, 8 GiB per FP32
-    transform; three passes) through size-independent checks on the device:
+    transform; three or four passes) through size-independent checks on the
+    device:
     an impulse at n0 gives X[k] = w_N^(k n0) (closed form, FP64 on the
     device), and forward out of place then backward in place returns N x."""
     torch = torch_dev
@@ -263,7 +264,7 @@ def test_largest_lengths(gpu, torch_dev, n, bits):
     x[n0] = 1.0
     y = torch.empty_like(x)
     with gpu.Plan(n, 1, bits, gpu.FORWARD) as p:
-        assert p.description().pass_count == 3
+        assert p.description().pass_count in (3, 4)
         p.execute_async(x, y, s)
         torch.cuda.synchronize()
     num = 0.0

@@ -110,10 +110,15 @@ def test_no_device_means_unsupported_hardware():
     (59049, 32, [243, 243]), (117649, 64, [343, 343]),
     (100000, 32, [400, 250]), (33600, 32, [192, 175]),
     (161051, 64, [121, 121, 11]), (8748, 64, [108, 81]),
+    # 1000-point FP32 strided kernels have 64-byte runs: three wide passes
+    (10 ** 6, 32, [100, 100, 100]), (10 ** 6, 64, [1000, 1000]),
+    # 2048-point passes cost three short ones
+    (3 << 21, 32, [384, 128, 128]),
 ])
 def test_model_schedule(n, bits, passes):
     """Lengths without a wisdom entry: the analytical planner's schedule
-    (fewest HBM passes, then the most balanced split)."""
+    (lowest weighted HBM cost, then fewest passes, then the most balanced
+    split)."""
     d = mf.describe(n, 1, bits)
     assert d.passes == passes
     assert math.prod(d.chain) == n
