@@ -857,12 +857,17 @@ static double scalar_median(double *v, size_t n) {
 }
 
 static double schedule_model_cost(const mofft_schedule *s) {
-  /* HBM round trips dominate: one read + one write per pass, plus a small
+  /* HBM round trips dominate: one read + one write per pass, each strided
+   * pass weighted by its measured cost (pass_weight), plus a small
    * per-stage compute term */
   const double bytes = 2.0 * (double)s->length * (double)s->batch *
                        (double)complex_bytes(s->precision_bits);
-  return (double)s->pass_count * bytes / 8.0e12 +
-         (double)s->depth * 1e-7;
+  double passes = 0.0;
+  for (int i = 0; i < s->pass_count; ++i) {
+    const mofft_registry_entry *e = &mofft_registry[s->passes[i].kernel];
+    passes += e->rows ? 1.0 : 0.1 * (double)pass_weight(e);
+  }
+  return passes * bytes / 8.0e12 + (double)s->depth * 1e-7;
 }
 
 static int same_candidate(const measured_candidate *a,
