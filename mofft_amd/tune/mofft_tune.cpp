@@ -290,7 +290,13 @@ int main(int argc, char **argv) {
         fprintf(stderr, "allocation failed for N=%zu\n", n);
         return 1;
       }
-      hipMemset(in, 0, n * batch * cb);
+      /* seeded uniform input, as the transforms will see: all-zero
+       * buffers time 10-15 % faster on MI355X for multi-pass plans */
+      if (mofft_server_generate(in, bits, n * batch, 2026, stream) != MOFFT_SUCCESS ||
+          hipStreamSynchronize(stream) != hipSuccess) {
+        fprintf(stderr, "input generation failed for N=%zu\n", n);
+        return 1;
+      }
       mofft_plan_options o;
       mofft_plan_options_init(&o);
       o.mode = MOFFT_PLANNER_MEASURE;
