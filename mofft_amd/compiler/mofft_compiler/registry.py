@@ -66,7 +66,10 @@ def smooth_lengths(limit: int = SMOOTH_MAX) -> tuple[int, ...]:
 # single-pass plans.
 ALT_RADIX4 = ((64, (4, 4, 4)), (256, (4, 4, 4, 4)), (512, (8, 4, 4, 4)),
               (1024, (4, 4, 4, 4, 4)), (2048, (8, 4, 4, 4, 4)),
-              (4096, (4, 4, 4, 4, 4, 4)))
+              (4096, (4, 4, 4, 4, 4, 4)),
+              # four sub-stages without a radix-2 one for the long single
+              # passes (the radix-16 chains end in 2 and 4)
+              (8192, (16, 8, 8, 8)), (16384, (16, 16, 8, 8)))
 
 
 @dataclass(frozen=True)
