@@ -26,8 +26,8 @@ Prints ONE JSON line:
              on a bounded sample of the same workload (FFTW 3.3.11, the
              comparison the reference uses, is not available offline).
 
-Other workloads (--workload c1|c3|c4) print their own lines for study; the
-driver's line is the default.  c4: 4096 FP32 transforms of N = 2^U{6..18}
+Other workloads (--workload c1|c3|c4|c5) print their own lines for study;
+the benchmark's result is the default workload's line.  c4: 4096 FP32 transforms of N = 2^U{6..18}
 (seeded), half forward, half backward, mixed placement, in one persistent
 launch replayed from a captured HIP graph, beside the same transforms
 launched one plan execution each.

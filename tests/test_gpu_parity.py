@@ -1,6 +1,6 @@
 """GPU parity: libmofft.so on gfx950 against the FP64 oracle.
 
-Bars (north star): plain FP32 relative L2 error <= 1e-6*log2(N), FP64
+Bars: plain FP32 relative L2 error <= 1e-6*log2(N), FP64
 <= 1e-15*log2(N), and forward->backward round trips back to N*x within the
 same bounds.  The oracle is FP64 direct summation (N <= 1024) or the
 compensated double-double FFT (larger N, and large batches), both pinned
