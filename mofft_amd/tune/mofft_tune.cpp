@@ -6,7 +6,8 @@
  * build: for every (length, precision) it runs the library's MEASURE planner
  * over the GPU candidate space (pass decompositions x pass boundaries x
  * kernel banks / tile shapes, timed with HIP events on a batch of transforms),
- * re-times the winner and the analytical model plan in an interleaved steady
+ * re-times the winner and the PRETUNED plan (the compiled wisdom's entry,
+ * else the analytical model's plan) in an interleaved steady
  * phase, and records the faster one.  The JSON goes through
  * mofft_compiler.wisdom into the library at the next build.
  *
@@ -16,9 +17,9 @@
  * kernel of the length, every split into two column-kernel passes whose
  * lengths multiply to it (and, above 2^20 points, into three passes of at
  * least 64 points), each under every plain tile bank, timed the same way;
- * the fastest of the MEASURE winner, the sweep winner and the model plan is
+ * the fastest of the MEASURE winner, the sweep winner and the PRETUNED plan is
  * recorded, after its output for a seeded input has been checked against
- * the model plan's (relative L2 within four times the precision's bound).
+ * the PRETUNED plan's (relative L2 within four times the precision's bound).
  *
  * usage: mofft_tune [--precision 32|64|both] [--lengths N,N,...]
  *                   [--min-log2 A --max-log2 B] [--points P]
@@ -101,7 +102,7 @@ double steady(int bits, void *p, void *in, void *out, hipStream_t s, int reps) {
 /* Relative L2 distance between two plans' outputs for one seeded input
  * (at most the first 2^20 points of the batch), so that a timing never
  * stands in for a result: a plan is recorded only when it agrees with the
- * analytical plan. */
+ * PRETUNED plan. */
 double disagreement(int bits, void *a, void *b, void *in, void *out, void *out2,
                     size_t n, size_t batch, hipStream_t s) {
   const size_t cb = bits == 32 ? 8 : 16;
@@ -350,7 +351,8 @@ int main(int argc, char **argv) {
       r.batch = batch;
       r.candidates = d.candidates_measured + swept_count;
       r.model_seconds = tm / 3;
-      /* the model plan unless a measured one is at least 1% faster */
+      /* the PRETUNED plan ("model": wisdom entry or analytical plan) unless
+       * a measured one is at least 1% faster */
       const mofft_plan_diagnostics *win = model ? &dm : &d;
       double tbest = model ? tm : tw;
       const char *which = model ? "model" : "measured";
