@@ -22,9 +22,26 @@ Prints ONE JSON line:
              time.  `traffic` is the
              measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE,
              profiles/), or null when no counter summary is present.
-  cpu_baseline  the reference algorithm restated in C (oracle/), one core,
-             on a bounded sample of the same workload (FFTW 3.3.11, the
-             comparison the reference uses, is not available offline).
+  protocol   the reference benchmark's sampling (tools/benchmark.cpp:65-130)
+             on the same step: warm-up of at least 1000 steps and 0.25 s,
+             repetitions sized so one sample takes about 2 ms, 31 samples;
+             median ms per step, 95 % bootstrap CI of the median (2000
+             resamples, seed 0x4d6f464654 + samples, :119-130) and the
+             coefficient of variation
+  cpu_baseline  the reference's executor (radix stages as DFT-matrix
+             contractions, transposes between stages; tables made at plan
+             time) restated in C and compiled for the host
+             (oracle/mofft_cpu_port.c), one core, on a bounded sample of the
+             same workload.  FFTW 3.3.11, the comparison the reference uses,
+             has no source offline ("fftw": "FFTW baseline unavailable");
+             beside it, clearly labelled, numpy/scipy's pocketfft on the
+             same sample, one thread and all host threads
+  configs    the other configurations of the workload set, measured in the
+             same run: c1 (FP32 N=1024 x 64, latency), c3 (FP32 N=2^4..2^20,
+             2^24 points each, plain and matrix-core form, fraction of the
+             HBM peak), c4 (4096 uneven FP32 transforms in one persistent
+             launch), c5 (the serving loop on a short Poisson window,
+             p50/p99 latency); --no-configs leaves them out
 
 Other workloads (--workload c1|c3|c4|c5) print their own lines for study;
 the benchmark's result is the default workload's line.  c4: 4096 FP32 transforms of N = 2^U{6..18}
@@ -68,6 +85,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--sweep-max", type=int, default=20)
+    ap.add_argument("--no-configs", action="store_true",
+                    help="default workload only: leave out the c1/c3/c4/c5 summaries")
+    ap.add_argument("--configs-serve-seconds", type=float, default=3.0,
+                    help="arrival window of the c5 summary in the default run")
+    ap.add_argument("--samples", type=int, default=31,
+                    help="protocol samples (tools/benchmark.cpp takes >= 7, 30 formal)")
     return ap.parse_args()
 
 
@@ -81,28 +104,69 @@ def percentile(values, q):
 
 
 def cpu_baseline_port(n, bits, seconds):
-    """Reference executor restated (oracle/), one thread, forward+backward of
-    single transforms of length n until `seconds` of CPU work are spent."""
+    """The reference executor on one host core (oracle/mofft_cpu_port.c:
+    DFT-matrix stages and transposes, coefficients and twiddles made at plan
+    time), forward then backward of single transforms of length n with the
+    GPU plan's radix chain, until `seconds` of CPU work are spent; the plans
+    are made before the timed loop, as the reference makes them once."""
     import numpy as np
     import oracle
     from mofft_amd import describe
     chain = describe(n, 1, bits).chain
+    dt = np.complex64 if bits == 32 else np.complex128
     rng = np.random.default_rng(7)
-    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    x = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(dt)
+    y = np.empty_like(x)
+    z = np.empty_like(x)
+    fwd = oracle.PortPlan(n, chain, -1, bits)
+    bwd = oracle.PortPlan(n, chain, 1, bits)
+    fwd.execute(x, y)
+    bwd.execute(y, z)
     count = 0
     t0 = time.perf_counter()
     while True:
-        y = oracle.ref_execute(x, chain, -1, bits)
-        oracle.ref_execute(y, chain, 1, bits)
+        fwd.execute(x, y)
+        bwd.execute(y, z)
         count += 1
         el = time.perf_counter() - t0
         if el >= seconds:
             break
+    fwd.destroy()
+    bwd.destroy()
     flops = 2 * 5.0 * n * math.log2(n) * count
     return {"value": round(flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": 1,
             "kind": "port",
             "sample": f"{count} FP{bits} transforms of N={n}, forward+backward, "
-                      f"radix chain {chain}, {el:.1f} s on one host core"}
+                      f"radix chain {chain}, DFT-matrix stages with tables made "
+                      f"at plan time (oracle/mofft_cpu_port.c), {el:.1f} s on one "
+                      f"host core"}
+
+
+def cpu_pocketfft(n, batch, bits, seconds):
+    """numpy/scipy's pocketfft (the FFTW stand-in, labelled as such) on the
+    same transforms: batches of `batch` forward+backward on one thread and
+    on every host thread (at most 16, the box's CPU share)."""
+    import numpy as np
+    import scipy.fft as sfft
+    dt = np.complex64 if bits == 32 else np.complex128
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))).astype(dt)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    out = {"library": "scipy.fft (pocketfft) " + sfft.__name__, "fftw": "FFTW baseline unavailable"}
+    for label, w in (("threads_1", 1), ("threads_all", threads)):
+        sfft.ifft(sfft.fft(x, workers=w), workers=w)
+        count = 0
+        t0 = time.perf_counter()
+        while True:
+            sfft.ifft(sfft.fft(x, workers=w), workers=w)
+            count += 1
+            el = time.perf_counter() - t0
+            if el >= seconds:
+                break
+        out[label] = {"value": round(2 * 5.0 * n * math.log2(n) * batch * count / el / 1e9, 3),
+                      "unit": "GFLOP/s", "threads": w,
+                      "sample": f"{count} x (batch {batch} FP{bits} N={n} forward+backward), {el:.1f} s"}
+    return out
 
 
 def load_traffic(kernel_key):
@@ -134,6 +198,52 @@ def measure_copy(torch, dev, stream, nbytes):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
     return round(2.0 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def protocol_samples(torch, stream, step, samples):
+    """tools/benchmark.cpp:65-130 on one step function: warm-up until at
+    least 1000 steps and 0.25 s (at most 10^6), repetitions per sample sized
+    to ~2 ms from the fastest single step, `samples` samples timed with HIP
+    events on the launch stream; median, bootstrap 95 % CI of the median
+    (2000 resamples, seed 0x4d6f464654 + samples) and the coefficient of
+    variation, in ms per step."""
+    import numpy as np
+    t0 = time.perf_counter()
+    warm = 0
+    while warm < 1000000 and (warm < 1000 or time.perf_counter() - t0 < 0.25):
+        step()
+        warm += 1
+        if warm % 64 == 0:
+            torch.cuda.synchronize()
+    torch.cuda.synchronize()
+    warm_s = time.perf_counter() - t0
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    fastest = float("inf")
+    for _ in range(5):
+        e0.record(stream)
+        step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        fastest = min(fastest, e0.elapsed_time(e1) * 1e-3)
+    reps = int(min(1000000, max(1, math.ceil(0.002 / fastest))))
+    per = []
+    for _ in range(samples):
+        e0.record(stream)
+        for _ in range(reps):
+            step()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        per.append(e0.elapsed_time(e1) / reps)
+    x = np.array(per)
+    rng = np.random.Generator(np.random.MT19937(0x4D6F464654 + len(per)))
+    boots = np.median(x[rng.integers(0, len(x), (2000, len(x)))], axis=1)
+    return {"median_ms_per_step": round(float(np.median(x)), 5),
+            "ci95_ms": [round(float(np.quantile(boots, 0.025)), 5),
+                        round(float(np.quantile(boots, 0.975)), 5)],
+            "cv": round(float(np.std(x) / np.mean(x)), 4),
+            "samples": len(per), "repetitions": reps,
+            "warmup_steps": warm, "warmup_s": round(warm_s, 3)}
 
 
 def run_gpu(args):
@@ -176,11 +286,14 @@ def run_gpu(args):
                           "bytes_written": nbytes, "copy_gbs": gbs}), flush=True)
         return None, None
     elif args.workload == "c4":
-        return run_batch(args)
+        print(json.dumps(run_batch(args)), flush=True)
+        return None, None
     elif args.workload == "c5":
-        return run_serve(args)
+        print(json.dumps(run_serve(args)), flush=True)
+        return None, None
     else:
-        return run_sweep(args)
+        run_sweep(args)
+        return None, None
 
     desc = plans[0][0].description()
     passes = desc.pass_count
@@ -241,6 +354,8 @@ def run_gpu(args):
     achieved = alg_per_launch / (launch_us * 1e-6) / 1e9
     traffic = load_traffic(kernel_key)
     copy_gbs = measure_copy(torch, dev, stream, 256 << 20)
+    proto = protocol_samples(torch, stream, step, args.samples)
+    proto["gflops_at_median"] = round(flops / (proto["median_ms_per_step"] * 1e-3) / 1e9, 2)
     line = {
         "metric": "batched C2C FFT GFLOP/s (5N log2N) + % HBM roofline; p50/p99 latency",
         "value": round(gflops, 2),
@@ -274,6 +389,7 @@ def run_gpu(args):
                      "avg_launch_us": round(launch_us, 3),
                      "pass_launch_us": [round(v, 3) for v in pass_us]},
         "wall_s": round(wall, 3),
+        "protocol": proto,
     }
     if args.workload == "c1":
         # the same execution replayed from a captured HIP graph
@@ -375,8 +491,7 @@ def run_batch(args):
     bp.destroy()
     for p in plans.values():
         p.destroy()
-    print(json.dumps(line), flush=True)
-    return None, None
+    return line
 
 
 def run_serve(args):
@@ -434,11 +549,10 @@ def run_serve(args):
         "resident_plans": st.resident_plans,
         "wall_s": round(wall, 2),
     }
-    print(json.dumps(line), flush=True)
-    return None, None
+    return line
 
 
-def run_sweep(args):
+def run_sweep(args, emit=True):
     """FP32 forward out-of-place, N = 2^k, batch = 2^24 / N: the plain FP32
     form and the matrix-core split-precision form (one line per N and
     form).  `rt_err` is the forward->backward round-trip relative L2 error
@@ -453,6 +567,7 @@ def run_sweep(args):
     x = torch.view_as_complex(torch.rand(total, 2, device=dev, generator=g) * 2 - 1)
     y = torch.empty_like(x)
     z = torch.empty_like(x)
+    rows = []
     for k in range(4, args.sweep_max + 1):
         n = 1 << k
         batch = total // n
@@ -475,15 +590,18 @@ def run_sweep(args):
             rt = (torch.linalg.vector_norm(z / n - x) / torch.linalg.vector_norm(x)).item()
             gf = 5.0 * n * k * batch / (ms * 1e-3) / 1e9
             gbs = 2.0 * total * 8 / (ms * 1e-3) / 1e9
-            print(json.dumps({"workload": "fp32 forward out-of-place sweep", "form": form,
-                              "length": n, "batch": batch, "passes": p.description().passes,
-                              "ms": round(ms, 4), "gflops": round(gf, 1),
-                              "hbm_gbs": round(gbs, 1),
-                              "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
-                              "rt_err": float(f"{rt:.3e}")}), flush=True)
+            row = {"workload": "fp32 forward out-of-place sweep", "form": form,
+                   "length": n, "batch": batch, "passes": p.description().passes,
+                   "ms": round(ms, 4), "gflops": round(gf, 1),
+                   "hbm_gbs": round(gbs, 1),
+                   "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                   "rt_err": float(f"{rt:.3e}")}
+            rows.append(row)
+            if emit:
+                print(json.dumps(row), flush=True)
             p.destroy()
             q.destroy()
-    return None, None
+    return rows
 
 
 def main():
@@ -496,11 +614,54 @@ def main():
     line, shape = run_gpu(args)
     if line is None:
         return
+    if args.workload == "c2" and not args.no_configs:
+        line["configs"] = run_configs(args)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_port(shape[0], shape[1], args.cpu_seconds)
+        line["cpu_pocketfft"] = cpu_pocketfft(shape[0], 16, shape[1], min(5.0, args.cpu_seconds))
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)
+
+
+def run_configs(args):
+    """The other configurations, measured in the same run and summarised
+    for the headline line (each progress line goes to stderr)."""
+    import argparse as ap
+    out = {}
+    c1 = ap.Namespace(**vars(args))
+    c1.workload = "c1"
+    line, _ = run_gpu(c1)
+    out["c1"] = {k: line[k] for k in ("value", "unit", "ms_per_step", "p50_ms", "p99_ms",
+                                      "graph_p50_ms", "graph_p99_ms", "hbm_gbs")}
+    out["c1"]["protocol"] = line["protocol"]
+    print("configs: c1 done", file=sys.stderr, flush=True)
+    rows = run_sweep(args, emit=False)
+    sweep = {}
+    for r in rows:
+        e = sweep.setdefault(r["length"], {"passes": r["passes"]})
+        e[f"{r['form']}_ms"] = r["ms"]
+        e[f"{r['form']}_hbm_frac"] = r["hbm_frac"]
+        e[f"{r['form']}_rt_err"] = r["rt_err"]
+    out["c3"] = {"workload": "fp32 forward out-of-place, N=2^k, batch 2^24/N",
+                 "per_length": {str(k): v for k, v in sorted(sweep.items())},
+                 "hbm_frac_ge_0.70": [n for n, v in sorted(sweep.items())
+                                      if v.get("plain_hbm_frac", 0) >= 0.70]}
+    print("configs: c3 done", file=sys.stderr, flush=True)
+    c4 = run_batch(args)
+    out["c4"] = {k: c4[k] for k in ("value", "unit", "ms_per_step", "p50_ms", "p99_ms",
+                                    "hbm_gbs", "speedup_vs_per_transform")}
+    out["c4"]["roofline_frac"] = c4["roofline"]["frac"]
+    print("configs: c4 done", file=sys.stderr, flush=True)
+    c5a = ap.Namespace(**vars(args))
+    c5a.serve_seconds = args.configs_serve_seconds
+    c5 = run_serve(c5a)
+    out["c5"] = {k: c5[k] for k in ("value", "unit", "p50_ms", "p99_ms", "completed",
+                                    "rejected", "completed_per_s", "offered_per_s",
+                                    "hbm_gbs", "peak_bytes_in_use_gb", "wall_s")}
+    out["c5"]["window_s"] = args.configs_serve_seconds
+    print("configs: c5 done", file=sys.stderr, flush=True)
+    return out
 
 
 if __name__ == "__main__":

@@ -83,9 +83,13 @@ mofft_status mofft_execute_async_workspace_f64(
     void *stream);
 
 /* A plan's launches on fixed device buffers captured into a HIP graph: one
- * replay per execution, for launch-bound loops.  The graph uses the plan's
- * twiddles and workspace (destroy it before the plan); replays of one graph
- * must be ordered on one stream. */
+ * replay per execution, for launch-bound loops.  The graph holds a
+ * reference to the plan's twiddle tables and workspace as they were at
+ * capture: a later setter that regroups the plan's passes gives the plan
+ * new tables while the graph keeps replaying the captured schedule on the
+ * old ones, and the graph stays valid after the plan is destroyed (the
+ * tables are freed with the last of the two).  Replays of one graph must be
+ * ordered on one stream. */
 typedef struct mofft_graph mofft_graph;
 mofft_status mofft_plan_graph_create_f32(const mofft_plan_f32 *plan,
                                          const mofft_complex_f32 *input,
@@ -153,6 +157,7 @@ typedef struct {
   int columns_per_block;
   int stage_count;
   int stages[8];       /* execution order: stage 0 first (innermost) */
+  int variant;         /* tile-shape family = the kernel bank selecting it */
 } mofft_kernel_info;
 int mofft_kernel_count(void);
 mofft_status mofft_kernel_get_info(int index, mofft_kernel_info *info);
@@ -327,7 +332,13 @@ mofft_status mofft_server_submit(mofft_server *server,
                                  uint64_t *ticket);
 /* Retire finished work, admit pending requests, and return up to
  * `capacity` new completions.  With wait != 0, block until at least one
- * completion is available or nothing is pending or in flight. */
+ * completion is available or nothing is pending or in flight.  When the
+ * pending head cannot be admitted while nothing is in flight - the arena
+ * space it needs is held by completions the caller has not released - a
+ * wait could never end: the call then returns MOFFT_ALLOCATION_FAILURE with
+ * *count = 0, and the caller releases completions and polls again.
+ * mofft_server_run_trace fails such a head (status
+ * MOFFT_ALLOCATION_FAILURE in its completion) instead of stalling. */
 mofft_status mofft_server_poll(mofft_server *server, mofft_completion *out,
                                size_t capacity, size_t *count, int wait);
 /* Return a completed request's buffers to the arena. */

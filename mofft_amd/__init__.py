@@ -116,7 +116,7 @@ class KernelInfo(ctypes.Structure):
     _fields_ = [("precision_bits", ctypes.c_int), ("length", ctypes.c_int),
                 ("rows", ctypes.c_int), ("threads_per_column", ctypes.c_int),
                 ("columns_per_block", ctypes.c_int), ("stage_count", ctypes.c_int),
-                ("stages", ctypes.c_int * 8)]
+                ("stages", ctypes.c_int * 8), ("variant", ctypes.c_int)]
 
 
 class BatchOptions(ctypes.Structure):

@@ -104,6 +104,14 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False,
     for p in headers:
         hdr.update(p.read_bytes())
     deps = hdr.hexdigest()
+    # device code includes only the ABI header, the kernel headers and the
+    # generated codelets: a change to the public or runtime headers does not
+    # recompile the kernels
+    hip_hdr = hashlib.sha256()
+    for p in [CSRC / "mofft_abi.h"] + sorted((CSRC / "hip").glob("*.h")) + \
+            sorted(GEN.glob("*.h")):
+        hip_hdr.update(p.read_bytes())
+    hip_deps = hip_hdr.hexdigest()
     bid = build_id()
     tasks = []
     objs = []
@@ -124,8 +132,11 @@ def build(force: bool = False, jobs: int | None = None, verbose: bool = False,
         tasks.append((cmd, src, obj))
         objs.append(obj)
     jobs = jobs or min(16, os.cpu_count() or 4)
+    # longest first: the kernel files dominate the build
+    tasks.sort(key=lambda t: -t[1].stat().st_size)
     with cf.ThreadPoolExecutor(max_workers=jobs) as pool:
-        futs = [pool.submit(_compile, cmd, src, obj, deps, force)
+        futs = [pool.submit(_compile, cmd, src, obj,
+                            hip_deps if src.suffix == ".hip" else deps, force)
                 for cmd, src, obj in tasks]
         for f in futs:
             msg = f.result()

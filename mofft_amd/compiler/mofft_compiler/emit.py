@@ -189,29 +189,36 @@ def emit_kernel_part(es: list[registry.Entry], indices: list[int]) -> str:
     lines = [HEADER, '#include "../hip/fft_pass.h"', "", "namespace mofft {"]
     for i in indices:
         e = es[i]
-        lines.append(f"int launch_entry_{i}(const mofft_pass_args *a, void *s) {{"
+        lines.append(f"int launch_{e.symbol}(const mofft_pass_args *a, void *s) {{"
                      f" return PassKernel<{_targs(e)}>::launch(a, (hipStream_t)s); }}")
         lines.append(f"static_assert(PassKernel<{_targs(e)}>::LDS == {e.lds_bytes},"
-                     f' "registry LDS bytes of entry {i}");')
+                     f' "registry LDS bytes of {e.symbol}");')
     lines += ["}  // namespace mofft", ""]
     return "\n".join(lines)
+
+
+def kernel_part_of(e: registry.Entry, parts: int) -> int:
+    """Source file of an entry's launcher: a hash of its symbol, so that
+    adding or removing registry entries recompiles only the files those
+    entries land in (a file holds ~70 instantiations, ~4 s each)."""
+    return int(hashlib.sha256(e.symbol.encode()).hexdigest()[:8], 16) % parts
 
 
 def emit_launch_table(es: list[registry.Entry]) -> str:
     lines = [HEADER, '#include "../mofft_abi.h"', "",
              "namespace mofft {"]
-    for i in range(len(es)):
-        lines.append(f"int launch_entry_{i}(const mofft_pass_args *, void *);")
+    for e in es:
+        lines.append(f"int launch_{e.symbol}(const mofft_pass_args *, void *);")
     lines += ["", "typedef int (*launch_fn)(const mofft_pass_args *, void *);",
               "extern const launch_fn launch_table[] = {"]
-    for i in range(len(es)):
-        lines.append(f"  launch_entry_{i},")
+    for e in es:
+        lines.append(f"  launch_{e.symbol},")
     lines += ["};", f"extern const int launch_table_size = {len(es)};",
               "}  // namespace mofft", ""]
     return "\n".join(lines)
 
 
-def emit(out_dir: str | Path, parts: int = 8) -> dict:
+def emit(out_dir: str | Path, parts: int = 24) -> dict:
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     es = registry.entries()
@@ -222,14 +229,12 @@ def emit(out_dir: str | Path, parts: int = 8) -> dict:
     files["mofft_registry.c"] = emit_registry_c(es, bs)
     for precision in (32, 64):
         files[f"mofft_batch_f{precision}.hip"] = emit_batch_part(bs, precision)
-    # balance parts by an estimate of compile cost (threads * stages)
-    order = sorted(range(len(es)), key=lambda i: -(es[i].length * len(es[i].stages)))
     buckets: list[list[int]] = [[] for _ in range(parts)]
-    loads = [0] * parts
-    for i in order:
-        k = loads.index(min(loads))
-        buckets[k].append(i)
-        loads[k] += es[i].length * len(es[i].stages) + 64
+    for i, e in enumerate(es):
+        buckets[kernel_part_of(e, parts)].append(i)
+    for stale in out.glob("mofft_kernels_p*.hip"):
+        if int(stale.stem.rsplit("_p", 1)[1]) >= parts:
+            stale.unlink()
     for k, idx in enumerate(buckets):
         files[f"mofft_kernels_p{k}.hip"] = emit_kernel_part(es, sorted(idx))
     files["mofft_launch_table.cpp"] = emit_launch_table(es)

@@ -295,7 +295,11 @@ def entries() -> list[Entry]:
         for k in range(7, cols_max.bit_length()):
             add(_make(precision, 1 << k, 0, _pow2_chain8(1 << k)))
         for length, chain in ALT_RADIX4:
-            add(_make(precision, length, 0, chain))
+            # strided passes stop at cols_max: a longer strided tile gives a
+            # thread 16+ butterflies of registers (the 16384-point ones took
+            # over ten minutes each to compile) and no plan uses one
+            if length <= cols_max:
+                add(_make(precision, length, 0, chain))
         for k in range(1, BATCH_COLS_MAX.bit_length()):
             add(_make(precision, 1 << k, 0, BATCH_CHAIN(precision, 1 << k)))
     return out

@@ -119,13 +119,16 @@ __device__ __forceinline__ void batch_tile(const mofft_batch_segment &sg,
 }
 
 /* Poll the finished-tile counter of segment `dep` until it reaches
- * `target`.  False when the error flag is (or gets) raised. */
+ * `target`.  False when the error flag is (or gets) raised.  The polls are
+ * relaxed atomic loads (no cache maintenance per poll); the caller issues
+ * one agent-scope acquire fence after a successful wait, which pairs with
+ * the producers' release and orders every later load of their data. */
 __device__ __forceinline__ bool batch_wait(const mofft_batch_header &h,
                                            int dep, unsigned target) {
-  unsigned int *done = h.counters + 2 + dep;
-  unsigned int *error = h.counters + 1;
+  unsigned int *done = h.counters + MOFFT_BATCH_DONE(dep);
+  unsigned int *error = h.counters + MOFFT_BATCH_ERROR;
   uint64_t polls = 0;
-  while (__hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) <
+  while (__hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
          target) {
     if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
       return false;
@@ -175,7 +178,8 @@ __device__ __forceinline__ void batch_loop(const mofft_batch_header &h) {
       if (threadIdx.x == 0) {
         int sseg = -1;
         const int64_t t = (int64_t)__hip_atomic_fetch_add(
-            h.counters, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            h.counters + MOFFT_BATCH_HEAD, 1u, __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
         if (t < h.total_items) {
           int lo = 0, hi = (int)h.segment_count - 1;
           while (lo < hi) {
@@ -223,7 +227,8 @@ __device__ __forceinline__ void batch_loop(const mofft_batch_header &h) {
         end = c.tiles;
       } else {
         if (signal && threadIdx.x == 0)
-          __hip_atomic_fetch_add(h.counters + 2 + seg, 1u, __ATOMIC_RELEASE,
+          __hip_atomic_fetch_add(h.counters + MOFFT_BATCH_DONE(seg), 1u,
+                                 __ATOMIC_RELEASE,
                                  __HIP_MEMORY_SCOPE_AGENT);
         active = false;
       }
@@ -273,7 +278,7 @@ struct BatchLauncher {
                     hipStream_t s) {
     const int rc = allow(lds_bytes);
     if (rc != 0) return rc;
-    const int64_t words = 2 + h->counter_count;
+    const int64_t words = MOFFT_BATCH_COUNTER_ARRAY(h->counter_count);
     const unsigned rb = (unsigned)((words + 255) / 256);
     hipLaunchKernelGGL(batch_reset_kernel<0>, dim3(rb), dim3(256), 0, s,
                        h->counters, words);

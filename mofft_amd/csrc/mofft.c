@@ -97,6 +97,23 @@ static size_t pass_table_bytes(int precision_bits) {
 static const int k_radices[] = {64, 32, 16, 15, 14, 13, 12, 11, 10,
                                 9, 8, 7, 6, 5, 4, 3, 2};
 
+/* Device tables of a plan: twiddles and the in-place workspace.  Shared,
+ * reference-counted, with the HIP graphs captured from the plan: a setter
+ * that regroups the passes gives the plan new tables, and a graph captured
+ * before keeps replaying on the old ones until it is destroyed. */
+typedef struct {
+  void *twiddles;
+  void *workspace;
+  unsigned int refs;
+} mofft_device_tables;
+
+static void tables_drop(mofft_device_tables *t) {
+  if (!t || --t->refs > 0) return;
+  mofft_hip_free(t->twiddles);
+  mofft_hip_free(t->workspace);
+  free(t);
+}
+
 typedef struct mofft_plan_common {
   mofft_schedule s;
   int direction;
@@ -104,6 +121,7 @@ typedef struct mofft_plan_common {
   int direct_input;
   mofft_stage_layout stage_layout;
   unsigned int kernel_bank;
+  mofft_device_tables *tables;  /* owner of the next two */
   void *twiddles;
   void *workspace;
   void *staging_in;
@@ -232,6 +250,7 @@ mofft_status mofft_kernel_get_info(int index, mofft_kernel_info *info) {
   info->columns_per_block = e->columns_per_block;
   info->stage_count = e->stage_count;
   for (int i = 0; i < e->stage_count && i < 8; ++i) info->stages[i] = e->stages[i];
+  info->variant = e->variant;
   return MOFFT_SUCCESS;
 }
 
@@ -568,8 +587,7 @@ const char *mofft_wisdom_source(void) { return mofft_generated_wisdom_source(); 
 
 static void plan_release(mofft_plan_common *p) {
   if (!p) return;
-  mofft_hip_free(p->twiddles);
-  mofft_hip_free(p->workspace);
+  tables_drop(p->tables);
   mofft_hip_free(p->staging_in);
   mofft_hip_free(p->staging_out);
   free(p);
@@ -621,8 +639,28 @@ static mofft_status fill_schedule_twiddles(const mofft_schedule *s,
   return MOFFT_SUCCESS;
 }
 
-static mofft_status fill_twiddles(mofft_plan_common *p) {
-  return fill_schedule_twiddles(&p->s, &p->twiddles);
+/* New device tables (twiddles and workspace) for p's schedule; the plan
+ * keeps its current ones on failure. */
+static mofft_status make_tables(mofft_plan_common *p, const mofft_schedule *s) {
+  mofft_device_tables *t =
+      (mofft_device_tables *)calloc(1, sizeof(mofft_device_tables));
+  if (!t) return MOFFT_ALLOCATION_FAILURE;
+  t->refs = 1;
+  mofft_status st = fill_schedule_twiddles(s, &t->twiddles);
+  if (st == MOFFT_SUCCESS) {
+    const size_t ws = schedule_workspace(s, p->placement);
+    if (ws && mofft_hip_malloc(&t->workspace, ws) != 0)
+      st = MOFFT_ALLOCATION_FAILURE;
+  }
+  if (st != MOFFT_SUCCESS) {
+    tables_drop(t);
+    return st;
+  }
+  tables_drop(p->tables);
+  p->tables = t;
+  p->twiddles = t->twiddles;
+  p->workspace = t->workspace;
+  return MOFFT_SUCCESS;
 }
 
 /* Build a plan from an explicit chain + mask (device resources included). */
@@ -643,12 +681,7 @@ static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
   p->direct_input = 0;
   p->stage_layout = MOFFT_STAGE_LAYOUT_TRANSPOSE;
   p->kernel_bank = bank;
-  st = fill_twiddles(p);
-  if (st == MOFFT_SUCCESS) {
-    const size_t ws = schedule_workspace(&p->s, placement);
-    if (ws && mofft_hip_malloc(&p->workspace, ws) != 0)
-      st = MOFFT_ALLOCATION_FAILURE;
-  }
+  st = make_tables(p, &p->s);
   if (st != MOFFT_SUCCESS) { plan_release(p); return st; }
   *out = p;
   return MOFFT_SUCCESS;
@@ -745,16 +778,25 @@ static int launch_chunk(const mofft_plan_common *p, const void *in, void *out,
   return 0;
 }
 
-static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
-                                 void *out, void *work, void *stream,
-                                 size_t count, launch_timing *timing) {
-  const mofft_schedule *s = &p->s;
-  for (size_t c0 = 0; c0 < count; c0 += s->chunk) {
-    const size_t cnt = count - c0 < s->chunk ? count - c0 : s->chunk;
+/* The first `count` transforms in chunks of at most `chunk` (the workspace
+ * `work` holds one chunk's intermediate). */
+static int launch_schedule_chunks(const mofft_plan_common *p, const void *in,
+                                  void *out, void *work, void *stream,
+                                  size_t count, size_t chunk,
+                                  launch_timing *timing) {
+  for (size_t c0 = 0; c0 < count; c0 += chunk) {
+    const size_t cnt = count - c0 < chunk ? count - c0 : chunk;
     const int rc = launch_chunk(p, in, out, work, stream, c0, cnt, timing);
     if (rc != 0) return rc;
   }
   return 0;
+}
+
+static int launch_schedule_timed(const mofft_plan_common *p, const void *in,
+                                 void *out, void *work, void *stream,
+                                 size_t count, launch_timing *timing) {
+  return launch_schedule_chunks(p, in, out, work, stream, count, p->s.chunk,
+                                timing);
 }
 
 static int launch_schedule(const mofft_plan_common *p, const void *in,
@@ -973,6 +1015,37 @@ static int candidate_time_compare(const void *l, const void *r) {
   return candidate_model_compare(l, r);
 }
 
+/* The MEASURE result check (the reference rejects a non-finite candidate
+ * output, src/mofft.c:725-737, 990-993): compares `count` complex values of
+ * a candidate's output with the same values of the analytical plan's.
+ * Returns 0 when every value is finite and the relative L2 difference is
+ * at most `bound`, 1 when a value is not finite, 2 when they disagree. */
+int mofft_internal_check_output(const void *out, const void *ref,
+                                size_t count, int precision_bits,
+                                double bound) {
+  double diff = 0.0, norm = 0.0;
+  for (size_t i = 0; i < 2 * count; ++i) {
+    const double a = precision_bits == 32 ? (double)((const float *)out)[i]
+                                          : ((const double *)out)[i];
+    if (!isfinite(a)) return 1;
+    if (!ref) continue;
+    const double b = precision_bits == 32 ? (double)((const float *)ref)[i]
+                                          : ((const double *)ref)[i];
+    diff += (a - b) * (a - b);
+    norm += b * b;
+  }
+  if (!ref) return 0;
+  return sqrt(diff) <= bound * sqrt(norm) ? 0 : 2;
+}
+
+/* Relative L2 error bound of one precision at length n: 1e-6 log2 n (FP32)
+ * or 1e-15 log2 n (FP64); two plans of one transform agree within twice
+ * that, and the search allows four times it. */
+static double precision_bound(size_t length, int precision_bits) {
+  const double l = log2((double)(length < 2 ? 2 : length));
+  return (precision_bits == 32 ? 1e-6 : 1e-15) * l;
+}
+
 static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
                                      size_t batch, int precision_bits,
                                      int direction, mofft_placement placement,
@@ -1014,6 +1087,9 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
   const size_t cb = complex_bytes(precision_bits);
   const size_t bytes = sat_mul(sat_mul(length, batch), cb);
   void *din = NULL, *dout = NULL, *dref = NULL;
+  void *hcheck = NULL, *href = NULL;
+  size_t check_count = 0, check_bytes = 0;
+  int have_ref = 0;
   mofft_status st = MOFFT_SUCCESS;
   if (mofft_hip_malloc(&din, bytes) || mofft_hip_malloc(&dout, bytes) ||
       (placement == MOFFT_IN_PLACE && mofft_hip_malloc(&dref, bytes))) {
@@ -1037,8 +1113,46 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
     if (rc) { st = MOFFT_INTERNAL_ERROR; goto done; }
     if (dref && mofft_hip_copy(dref, din, bytes)) { st = MOFFT_INTERNAL_ERROR; goto done; }
   }
+  /* the result check covers the first transforms of the batch (at most
+   * 8 MiB of output, at least one transform) */
+  check_count = (size_t)8 << 20;
+  check_count = check_count / (length * cb) < batch ? check_count / (length * cb)
+                                                     : batch;
+  if (check_count == 0) check_count = 1;
+  check_bytes = check_count * length * cb;
+  hcheck = malloc(check_bytes);
+  href = malloc(check_bytes);
+  if (!hcheck || !href) { st = MOFFT_ALLOCATION_FAILURE; goto done; }
+  {
+    /* reference output: the analytical plan on the same input */
+    mofft_plan_common *p = NULL;
+    if (depth > 0 &&
+        plan_from_chain(&p, length, batch, precision_bits, direction, placement,
+                        radices, depth, mask, 0) == MOFFT_SUCCESS) {
+      const void *xin = placement == MOFFT_IN_PLACE ? dout : din;
+      st = MOFFT_SUCCESS;
+      if (placement == MOFFT_IN_PLACE && mofft_hip_copy(dout, dref, bytes))
+        st = MOFFT_INTERNAL_ERROR;
+      if (st == MOFFT_SUCCESS)
+        st = execute_device(p, xin, dout, p->workspace,
+                            schedule_workspace(&p->s, placement), NULL);
+      if (st == MOFFT_SUCCESS && (mofft_hip_synchronize(NULL) ||
+                                  mofft_hip_copy(href, dout, check_bytes)))
+        st = MOFFT_INTERNAL_ERROR;
+      plan_release(p);
+      if (st != MOFFT_SUCCESS) goto done;
+      if (mofft_internal_check_output(href, NULL, check_count * length,
+                                      precision_bits, 0.0) != 0) {
+        st = MOFFT_INTERNAL_ERROR;
+        goto done;
+      }
+      have_ref = 1;
+    }
+  }
   {
     const size_t iterations = length * batch < 65536 ? 8 : (length * batch < (1u << 22) ? 4 : 1);
+    const double bound = 4.0 * precision_bound(length, precision_bits);
+    size_t accepted = 0;
     for (size_t i = 0; i < selected; ++i) {
       measured_candidate *c = &pool[i];
       mofft_plan_common *p = NULL;
@@ -1046,17 +1160,36 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
                            placement, c->radices, c->depth, c->mask, c->bank);
       if (st != MOFFT_SUCCESS) goto done;
       const void *xin = placement == MOFFT_IN_PLACE ? dout : din;
-      if (placement == MOFFT_IN_PLACE) mofft_hip_copy(dout, dref, bytes);
-      st = execute_device(p, xin, dout, p->workspace,
-                          schedule_workspace(&p->s, placement), NULL);
-      if (st == MOFFT_SUCCESS && mofft_hip_synchronize(NULL))
+      if (placement == MOFFT_IN_PLACE && mofft_hip_copy(dout, dref, bytes))
         st = MOFFT_INTERNAL_ERROR;
+      if (st == MOFFT_SUCCESS)
+        st = execute_device(p, xin, dout, p->workspace,
+                            schedule_workspace(&p->s, placement), NULL);
+      if (st == MOFFT_SUCCESS && (mofft_hip_synchronize(NULL) ||
+                                  mofft_hip_copy(hcheck, dout, check_bytes)))
+        st = MOFFT_INTERNAL_ERROR;
+      if (st == MOFFT_SUCCESS) {
+        /* a non-finite output fails the search (as the reference does); an
+         * output that disagrees with the analytical plan's drops the
+         * candidate: a timing is never taken for a result */
+        const int chk = mofft_internal_check_output(
+            hcheck, have_ref ? href : NULL, check_count * length,
+            precision_bits, bound);
+        if (chk == 1) st = MOFFT_INTERNAL_ERROR;
+        if (chk == 2) {
+          plan_release(p);
+          c->median = DBL_MAX;
+          continue;
+        }
+      }
       for (uint32_t smp = 0; smp < options->timing_samples && st == MOFFT_SUCCESS; ++smp) {
         double times[8];
         for (size_t it = 0; it < iterations && st == MOFFT_SUCCESS; ++it) {
           if (placement == MOFFT_IN_PLACE) {
-            mofft_hip_copy(dout, dref, bytes);
-            mofft_hip_synchronize(NULL);
+            if (mofft_hip_copy(dout, dref, bytes) || mofft_hip_synchronize(NULL)) {
+              st = MOFFT_INTERNAL_ERROR;
+              break;
+            }
           }
           /* device time of the launches (HIP events on the null stream) */
           double secs = 0.0;
@@ -1073,7 +1206,9 @@ static mofft_status search_best_plan(mofft_plan_common **out, size_t length,
       plan_release(p);
       if (st != MOFFT_SUCCESS) goto done;
       c->median = scalar_median(c->samples, options->timing_samples);
+      accepted++;
     }
+    if (accepted == 0) { st = MOFFT_INTERNAL_ERROR; goto done; }
   }
   qsort(pool, selected, sizeof(pool[0]), candidate_time_compare);
   st = plan_from_chain(out, length, batch, precision_bits, direction, placement,
@@ -1089,6 +1224,8 @@ done:
   mofft_hip_free(din);
   mofft_hip_free(dout);
   mofft_hip_free(dref);
+  free(hcheck);
+  free(href);
   free(pool);
   return st;
 }
@@ -1521,26 +1658,10 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
     p->kernel_bank = bank;
     return MOFFT_SUCCESS;
   }
-  mofft_plan_common tmp = *p;
-  tmp.s = s;
-  tmp.twiddles = NULL;
-  tmp.workspace = NULL;
-  st = fill_twiddles(&tmp);
-  if (st == MOFFT_SUCCESS) {
-    const size_t ws = schedule_workspace(&s, p->placement);
-    if (ws && mofft_hip_malloc(&tmp.workspace, ws) != 0)
-      st = MOFFT_ALLOCATION_FAILURE;
-  }
-  if (st != MOFFT_SUCCESS) {
-    mofft_hip_free(tmp.twiddles);
-    mofft_hip_free(tmp.workspace);
-    return st;
-  }
-  mofft_hip_free(p->twiddles);
-  mofft_hip_free(p->workspace);
+  /* graphs captured earlier keep their reference to the old tables */
+  st = make_tables(p, &s);
+  if (st != MOFFT_SUCCESS) return st;
   p->s = s;
-  p->twiddles = tmp.twiddles;
-  p->workspace = tmp.workspace;
   p->kernel_bank = bank;
   return MOFFT_SUCCESS;
 }
@@ -2084,6 +2205,38 @@ mofft_status mofft_internal_fill_twiddles(const mofft_schedule *s,
   return fill_schedule_twiddles(s, twiddles);
 }
 
+/* One device twiddle table, generated by the plan-time kernels and copied
+ * to `host` (count complex values), for a bitwise check against correctly
+ * rounded values.  kind 0: out[i] = w_a^((i * b) mod a), i < count (the
+ * two-level pass tables); kind 1: the sub-stage table of ns = a, radix b
+ * (count = a * (b - 1)); kind 2: the full pass table of ns = a, L = b
+ * (count = a * b). */
+mofft_status mofft_internal_twiddle_table(int kind, int precision_bits,
+                                          int64_t a, int64_t b, void *host,
+                                          size_t count) {
+  if (!host || count == 0 || a < 1 || b < 1 ||
+      (precision_bits != 32 && precision_bits != 64))
+    return MOFFT_INVALID_ARGUMENT;
+  if ((kind == 1 && count != (size_t)(a * (b - 1))) ||
+      (kind == 2 && count != (size_t)(a * b)) || kind < 0 || kind > 2)
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
+  const size_t bytes = count * complex_bytes(precision_bits);
+  void *dev = NULL;
+  if (mofft_hip_malloc(&dev, bytes) != 0) return MOFFT_ALLOCATION_FAILURE;
+  int rc;
+  if (kind == 0)
+    rc = mofft_hip_fill_powers(dev, precision_bits, (int64_t)count, a, b, NULL);
+  else if (kind == 1)
+    rc = mofft_hip_fill_stage(dev, precision_bits, a, (int)b, NULL);
+  else
+    rc = mofft_hip_fill_pass(dev, precision_bits, a, b, NULL);
+  if (rc == 0) rc = mofft_hip_synchronize(NULL);
+  if (rc == 0) rc = mofft_hip_copy(host, dev, bytes);
+  mofft_hip_free(dev);
+  return rc == 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
+}
+
 size_t mofft_internal_complex_bytes(int precision_bits) {
   return complex_bytes(precision_bits);
 }
@@ -2176,6 +2329,7 @@ mofft_status mofft_execute_timed_f64(const mofft_plan_f64 *plan,
 
 struct mofft_graph {
   void *exec;
+  mofft_device_tables *tables;  /* the plan tables the captured launches read */
 };
 
 static mofft_status graph_create(const mofft_plan_common *p, const void *in,
@@ -2205,6 +2359,8 @@ static mofft_status graph_create(const mofft_plan_common *p, const void *in,
     free(g);
     return MOFFT_INTERNAL_ERROR;
   }
+  g->tables = p->tables;
+  if (g->tables) g->tables->refs++;
   *graph = g;
   return MOFFT_SUCCESS;
 }
@@ -2232,5 +2388,6 @@ mofft_status mofft_graph_launch(const mofft_graph *graph, void *stream) {
 void mofft_graph_destroy(mofft_graph *graph) {
   if (!graph) return;
   mofft_hip_graph_destroy(graph->exec);
+  tables_drop(graph->tables);
   free(graph);
 }

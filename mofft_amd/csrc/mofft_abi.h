@@ -170,11 +170,20 @@ typedef struct {
   uint32_t reserved;
 } mofft_batch_segment;
 
-/* Kernel argument of the persistent batch kernel.  counters[0] is the
- * queue head, counters[1] the error flag (nonzero: a wait exceeded
- * spin_limit polls and the launch gave up), counters[2 + s] the finished
- * items of segment s (s < counter_count).  All of them are cleared by the
- * launch. */
+/* Kernel argument of the persistent batch kernel.  Every counter has a
+ * 128-byte line of its own (MOFFT_BATCH_COUNTER_WORDS words apart), so the
+ * queue head that every dequeue hits, the error flag that waits read and
+ * the per-segment counters that producers release never share a line:
+ * word MOFFT_BATCH_HEAD is the queue head, MOFFT_BATCH_ERROR the error flag
+ * (nonzero: a wait exceeded spin_limit polls and the launch gave up),
+ * MOFFT_BATCH_DONE(s) the finished items of segment s (s < counter_count).
+ * The array holds MOFFT_BATCH_COUNTER_ARRAY(counter_count) words; the
+ * launch clears all of them. */
+#define MOFFT_BATCH_COUNTER_WORDS 32
+#define MOFFT_BATCH_HEAD 0
+#define MOFFT_BATCH_ERROR MOFFT_BATCH_COUNTER_WORDS
+#define MOFFT_BATCH_DONE(s) (((int64_t)(s) + 2) * MOFFT_BATCH_COUNTER_WORDS)
+#define MOFFT_BATCH_COUNTER_ARRAY(count) MOFFT_BATCH_DONE(count)
 typedef struct {
   const mofft_batch_segment *segments;
   unsigned int *counters;

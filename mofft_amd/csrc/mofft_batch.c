@@ -568,7 +568,9 @@ mofft_status mofft_batch_plan_create(mofft_batch_plan **plan,
   st = build_queue(p, order, workers);
   if (st != MOFFT_SUCCESS) goto fail;
   if (mofft_hip_malloc(&p->d_segs, p->seg_count * sizeof(mofft_batch_segment)) != 0 ||
-      mofft_hip_malloc(&p->d_counters, (2 + p->queue_count) * sizeof(unsigned int)) != 0) {
+      mofft_hip_malloc(&p->d_counters,
+                       (size_t)MOFFT_BATCH_COUNTER_ARRAY(p->queue_count) *
+                           sizeof(unsigned int)) != 0) {
     st = MOFFT_ALLOCATION_FAILURE;
     goto fail;
   }
@@ -618,10 +620,11 @@ mofft_status mofft_batch_execute_async(const mofft_batch_plan *plan,
 
 mofft_status mofft_batch_check(const mofft_batch_plan *plan) {
   if (!plan) return MOFFT_INVALID_ARGUMENT;
-  unsigned int flag[2] = {0, 0};
-  if (mofft_hip_copy(flag, plan->d_counters, sizeof(flag)) != 0)
+  unsigned int flag = 0;
+  if (mofft_hip_copy(&flag, (unsigned int *)plan->d_counters + MOFFT_BATCH_ERROR,
+                     sizeof(flag)) != 0)
     return MOFFT_INTERNAL_ERROR;
-  return flag[1] == 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
+  return flag == 0 ? MOFFT_SUCCESS : MOFFT_INTERNAL_ERROR;
 }
 
 mofft_status mofft_batch_execute(const mofft_batch_plan *plan) {

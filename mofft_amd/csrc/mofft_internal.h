@@ -85,6 +85,18 @@ int mofft_internal_wisdom_lookup(size_t length, int precision_bits,
 mofft_status mofft_internal_fill_twiddles(const mofft_schedule *s,
                                           void **twiddles);
 size_t mofft_internal_complex_bytes(int precision_bits);
+/* A device-generated twiddle table copied to the host (kind 0: powers
+ * w_a^((i*b) mod a); 1: sub-stage table of ns = a, radix b; 2: full pass
+ * table of ns = a, L = b). */
+mofft_status mofft_internal_twiddle_table(int kind, int precision_bits,
+                                          int64_t a, int64_t b, void *host,
+                                          size_t count);
+/* MEASURE's result check on host copies of `count` complex values: 0 when
+ * all are finite and (ref != NULL) within relative L2 `bound` of ref, 1 when
+ * a value is not finite, 2 when they disagree. */
+int mofft_internal_check_output(const void *out, const void *ref,
+                                size_t count, int precision_bits,
+                                double bound);
 void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
                              mofft_plan_description *d);
 

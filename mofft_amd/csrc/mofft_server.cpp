@@ -403,6 +403,22 @@ mofft_status do_submit(mofft_server *s, const mofft_request *r, double t,
   return MOFFT_SUCCESS;
 }
 
+/* True when pending requests exist, nothing is in flight and admission
+ * (just run) admitted none of them: the head waits for arena space that no
+ * running execution will return. */
+bool admission_stalled(const mofft_server *s) {
+  return !s->pending.empty() && s->inflight.empty() && s->ready.empty();
+}
+
+/* Fail the pending head with `status` (its completion goes to `ready`). */
+void fail_head(mofft_server *s, int status) {
+  Job j = s->pending.front();
+  s->pending.pop_front();
+  j.block = 0;
+  j.start = now_seconds();
+  finish_job(s, j, j.start, status);
+}
+
 }  // namespace
 
 extern "C" {
@@ -469,6 +485,13 @@ mofft_status mofft_server_poll(mofft_server *s, mofft_completion *out,
       s->ready.pop_front();
     }
     if (n > 0 || !wait || (s->pending.empty() && s->inflight.empty())) break;
+    /* Nothing in flight and the head still not admitted: only the arena
+     * blocks it, and only a release of held completions can free arena
+     * space, so waiting would never end. */
+    if (admission_stalled(s)) {
+      if (count) *count = 0;
+      return MOFFT_ALLOCATION_FAILURE;
+    }
     sched_yield();
   }
   if (count) *count = n;
@@ -546,6 +569,14 @@ mofft_status mofft_server_run_trace(mofft_server *s,
       }
       mofft_server_release(s, c.ticket);
       finished++;
+      progress = true;
+    }
+    /* A head that cannot be admitted with nothing in flight waits for
+     * memory held by completions the caller took before this call; none of
+     * the trace's own requests can free it, so it fails instead of
+     * stalling the replay. */
+    if (!progress && admission_stalled(s)) {
+      fail_head(s, MOFFT_ALLOCATION_FAILURE);
       progress = true;
     }
     if (!progress) sched_yield();

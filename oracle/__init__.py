@@ -15,6 +15,9 @@ Functions follow ``oracle/mofft_oracle.c``:
 * :func:`omega` - exp(sign*2*pi*i*m/M) rounded once (src/mofft.c:1082-1140
   fills the same values).
 * :func:`plan_bytes` - byte accounting of src/mofft.c:190-304.
+* :class:`PortPlan` - the reference executor on one host core with its
+  tables made at plan time (``oracle/mofft_cpu_port.c``), timed by bench.py's
+  cpu_baseline.
 """
 from __future__ import annotations
 
@@ -59,6 +62,9 @@ def _load():
                                      ctypes.c_int, fp]
     lib.oracle_omega_dd.argtypes = [ctypes.c_uint64, ctypes.c_uint64,
                                     ctypes.c_int, dp, dp]
+    lib.oracle_omega_many.argtypes = [ctypes.POINTER(ctypes.c_uint64), sz,
+                                      ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_void_p]
     lib.oracle_plan_bytes.argtypes = [sz, ctypes.c_int, ip, sz,
                                       ctypes.c_uint64, ctypes.c_int,
                                       ctypes.c_int, sz,
@@ -143,21 +149,68 @@ def omega_table(big_m: int, sign: int = -1, precision_bits: int = 64,
     """Vector of omega values for m in `indices` (default range(M))."""
     lib = _load()
     if indices is None:
-        indices = range(big_m)
-    indices = list(indices)
-    if precision_bits == 32:
-        out = np.empty(len(indices), dtype=np.complex64)
-        buf = (ctypes.c_float * 2)()
-        for i, m in enumerate(indices):
-            lib.oracle_omega_f32(int(m), big_m, sign, buf)
-            out[i] = complex(buf[0], buf[1])
-        return out
-    out = np.empty(len(indices), dtype=np.complex128)
-    buf = (ctypes.c_double * 2)()
-    for i, m in enumerate(indices):
-        lib.oracle_omega_f64(int(m), big_m, sign, buf)
-        out[i] = complex(buf[0], buf[1])
+        indices = np.arange(big_m, dtype=np.uint64)
+    m = np.ascontiguousarray(np.asarray(indices, dtype=np.uint64))
+    out = np.empty(m.size, dtype=np.complex64 if precision_bits == 32 else np.complex128)
+    lib.oracle_omega_many(m.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), m.size,
+                          big_m, sign, precision_bits, out.ctypes.data_as(ctypes.c_void_p))
     return out
+
+
+_port = None
+
+
+def _load_port():
+    global _port
+    if _port is not None:
+        return _port
+    path = os.path.join(_HERE, "build", "libmofft_cpu_port.so")
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    for suf in ("f32", "f64"):
+        c = getattr(lib, f"port_create_{suf}")
+        c.argtypes = [ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                      ctypes.c_size_t]
+        c.restype = ctypes.c_void_p
+        e = getattr(lib, f"port_execute_{suf}")
+        e.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        e.restype = ctypes.c_int
+        getattr(lib, f"port_destroy_{suf}").argtypes = [ctypes.c_void_p]
+    _port = lib
+    return lib
+
+
+class PortPlan:
+    """The reference executor on one host core with its tables made at plan
+    time (oracle/mofft_cpu_port.c): bench.py's timed CPU baseline."""
+
+    def __init__(self, n: int, radices, sign: int = -1, precision_bits: int = 64):
+        self._lib = _load_port()
+        self._suf = "f32" if precision_bits == 32 else "f64"
+        self.dtype = np.complex64 if precision_bits == 32 else np.complex128
+        self.n = int(n)
+        r = np.ascontiguousarray(radices, dtype=np.int32)
+        self._h = getattr(self._lib, f"port_create_{self._suf}")(
+            self.n, int(sign), r.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), r.size)
+        if not self._h:
+            raise ValueError(f"port rejected n={n} radices={list(radices)}")
+
+    def execute(self, x: np.ndarray, out: np.ndarray) -> None:
+        assert x.dtype == self.dtype and out.dtype == self.dtype
+        assert x.size == self.n and out.size == self.n
+        rc = getattr(self._lib, f"port_execute_{self._suf}")(
+            self._h, x.ctypes.data, out.ctypes.data)
+        if rc:
+            raise ValueError("port execute failed")
+
+    def destroy(self) -> None:
+        if self._h:
+            getattr(self._lib, f"port_destroy_{self._suf}")(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.destroy()
 
 
 def plan_bytes(length: int, precision_bits: int, radices, blocked_mask=0,

@@ -81,6 +81,14 @@ void oracle_omega_f32(uint64_t m, uint64_t M, int sign, float out[2]) {
   out[1] = (float)(sign < 0 ? -s : s);
 }
 
+void oracle_omega_many(const uint64_t *m, size_t count, uint64_t M, int sign,
+                       int precision_bits, void *out) {
+  for (size_t i = 0; i < count; ++i) {
+    if (precision_bits == 32) oracle_omega_f32(m[i], M, sign, (float *)out + 2 * i);
+    else oracle_omega_f64(m[i], M, sign, (double *)out + 2 * i);
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* Double-double arithmetic (Dekker / Knuth error-free transforms)       */
 /* ------------------------------------------------------------------ */

@@ -25,6 +25,9 @@ void oracle_omega_dd(uint64_t m, uint64_t M, int sign, double out_re[2],
 /* Same value rounded once to double / to float. */
 void oracle_omega_f64(uint64_t m, uint64_t M, int sign, double out[2]);
 void oracle_omega_f32(uint64_t m, uint64_t M, int sign, float out[2]);
+/* out[i] = omega(m[i], M, sign) rounded once to FP32 or FP64, i < count. */
+void oracle_omega_many(const uint64_t *m, size_t count, uint64_t M, int sign,
+                       int precision_bits, void *out);
 
 /* Direct O(N^2) DFT with double-double accumulation and double-double
  * twiddles; result rounded once to double. */

@@ -61,13 +61,14 @@ def gpu(mofft):
 _SM_GAMMA = np.uint64(0x9E3779B97F4A7C15)
 
 
-def seeded_device_uniform(seed, count, bits):
+def seeded_device_uniform(seed, count, bits, start=0):
     """Host restatement of the library's device input generator
     (mofft_hip_fill_uniform / mofft_server_generate): component c of complex
     element i is 2u - 1, u = top 53 bits of
-    splitmix64(seed + (2i + c + 1) * 0x9E3779B97F4A7C15) * 2^-53."""
+    splitmix64(seed + (2i + c + 1) * 0x9E3779B97F4A7C15) * 2^-53.  Returns
+    elements start .. start + count - 1."""
     with np.errstate(over="ignore"):
-        k = np.arange(1, 2 * count + 1, dtype=np.uint64)
+        k = np.arange(2 * start + 1, 2 * (start + count) + 1, dtype=np.uint64)
         z = np.uint64(seed) + k * _SM_GAMMA
         z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
         z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)

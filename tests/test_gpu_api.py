@@ -105,12 +105,19 @@ def test_pass_grouping_agrees(gpu, bits):
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("n", [1024, 1 << 16])
 def test_every_kernel_bank(gpu, n, bits):
+    """Every kernel bank on the same chain and pass grouping.  Banks 0, 1, 2
+    and 4 change only the tile shape (columns per workgroup): the same
+    arithmetic on the same values, so their outputs are bit-identical (the
+    GPU form of the reference's linear-vs-blocked transpose identity,
+    tests/test_fft.c:253-306).  Bank 3 (matrix-core split precision, FP32)
+    rounds differently and is checked against the oracle bar only."""
     x = seeded_complex((4, n), seed=n).astype(np.complex64 if bits == 32 else np.complex128)
     ref = oracle.fft_dd(x, -1)
     with gpu.Plan(n, 4, bits) as p0:
         count = p0.diagnostics.kernel_bank_count
     assert count >= 2
     L = gpu.lib()
+    outs = {}
     for bank in range(count):
         with gpu.Plan(n, 4, bits) as p:
             assert getattr(L, f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, bank) == 0
@@ -120,6 +127,10 @@ def test_every_kernel_bank(gpu, n, bits):
             y = np.empty_like(x)
             p.execute(x, y)
         assert oracle.rel_l2(y, ref) <= tol(n, bits)
+        outs[bank] = y
+    for bank, y in outs.items():
+        if bank != gpu.MFMA_BANK:
+            assert np.array_equal(y, outs[0]), ("bank", bank, "differs from bank 0")
     with gpu.Plan(n, 4, bits) as p:
         assert getattr(L, f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, count) == gpu.INVALID_ARGUMENT
 
@@ -177,6 +188,61 @@ def test_strict_wisdom_and_model_fallback(gpu):
     # 1024 has one: STRICT succeeds and reports the wisdom plan
     with gpu.Plan(1024, 1, 32, flags=gpu.STRICT_WISDOM) as p:
         assert p.diagnostics.used_wisdom == 1 and p.diagnostics.used_model_fallback == 0
+
+
+def _device_table(gpu, kind, bits, a, b, count):
+    out = np.empty(count, np.complex64 if bits == 32 else np.complex128)
+    st = gpu.lib().mofft_internal_twiddle_table(
+        ctypes.c_int(kind), ctypes.c_int(bits), ctypes.c_int64(a), ctypes.c_int64(b),
+        out.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(count))
+    assert st == gpu.SUCCESS
+    return out
+
+
+def _ulps(got, want):
+    """Largest distance in units of the last place of `want`, per part."""
+    got = np.asarray(got)
+    want = np.asarray(want)
+    ft = np.float32 if got.dtype == np.complex64 else np.float64
+    worst = 0.0
+    for a, b in ((got.real, want.real), (got.imag, want.imag)):
+        sp = np.spacing(np.abs(b).astype(ft))
+        worst = max(worst, float(np.max(np.abs(a.astype(np.float64) - b.astype(np.float64)) / sp)))
+    return worst
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("kind,a,b", [(0, 1 << 20, 1), (0, 1 << 16, 1 << 6), (0, 3 * 5 * 7 * 11 * 13 * 16, 7),
+                                      (1, 256, 16), (1, 729, 3), (1, 1000, 12),
+                                      (2, 128, 512), (2, 12, 1728), (2, 1024, 1024)])
+def test_twiddle_tables_bitwise(gpu, bits, kind, a, b):
+    """Every twiddle table the plans use is generated on the device in FP64
+    and rounded once (src/mofft.c:1097-1105 rounds cos/sin of a double
+    angle once).  Each table kind - the two-level powers w_M^((i*step) mod M),
+    the sub-stage table w_(ns*r)^(a*k) and the full pass table
+    w_(ns*L)^(a*k) - is compared with the oracle's long-double values
+    rounded once: FP32 tables are equal bit for bit, FP64 ones within one
+    ulp (the device sincospi is not correctly rounded)."""
+    if kind == 0:
+        count = min(a, 1 << 20)
+        m = (np.arange(count, dtype=np.uint64) * np.uint64(b)) % np.uint64(a)
+        big_m = a
+    elif kind == 1:
+        count = a * (b - 1)
+        i = np.arange(count, dtype=np.uint64)
+        m = (i // np.uint64(b - 1)) * (i % np.uint64(b - 1) + np.uint64(1))
+        big_m = a * b
+    else:
+        count = a * b
+        i = np.arange(count, dtype=np.uint64)
+        m = (i % np.uint64(a)) * (i // np.uint64(a))
+        big_m = a * b
+    got = _device_table(gpu, kind, bits, a, b, count)
+    want = oracle.omega_table(big_m, -1, bits, indices=m % np.uint64(big_m))
+    if bits == 32:
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), _ulps(got, want)
+    else:
+        assert _ulps(got, want) <= 1.0
 
 
 @pytest.mark.parametrize("bits", [32, 64])
@@ -286,7 +352,7 @@ def test_largest_lengths(gpu, torch_dev, n, bits):
         b.execute_async(y, y, s)
         torch.cuda.synchronize()
     rt = (torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x)).item()
-    assert rt <= 2 * tol(n, bits), ("round trip", n, bits, rt)
+    assert rt <= tol(n, bits), ("round trip", n, bits, rt)
 
 
 @pytest.mark.parametrize("n,batch,bits", [(20736, 1000, 64), (1 << 21, 20, 32),
@@ -318,4 +384,46 @@ def test_multi_chunk_batches(gpu, torch_dev, n, batch, bits):
         b.execute_async(y, y, s)
         torch.cuda.synchronize()
     rt = (torch.linalg.vector_norm(y / n - x) / torch.linalg.vector_norm(x)).item()
-    assert rt <= 2 * tol(n, bits), ("round trip", rt)
+    assert rt <= tol(n, bits), ("round trip", rt)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_graph_outlives_regroup_and_plan(gpu, torch_dev, bits):
+    """A captured graph keeps the twiddle tables and workspace it was
+    captured with: after a setter regroups the plan's passes (new tables for
+    the plan) and after the plan is destroyed, replays still equal the
+    direct execution of the original grouping."""
+    torch = torch_dev
+    n, batch = 1 << 16, 4
+    x = seeded_complex((batch, n), seed=77, dtype=np.complex64 if bits == 32 else np.complex128)
+    src = to_dev(torch, x, bits)
+    buf = src.clone()
+    L = gpu.lib()
+    p = gpu.Plan(n, batch, bits, gpu.FORWARD, in_place=True, radices=[16, 16, 16, 16], pass_mask=2)
+    try:
+        p.execute_async(buf, buf, 0)
+        torch.cuda.synchronize()
+        want = buf.clone()
+        g = p.capture(buf, buf)
+        # one pass per radix: a different schedule, new tables for the plan
+        p.set_blocked_transpose_mask(7)
+        assert p.description().pass_count == 4
+        buf.copy_(src)
+        p.execute_async(buf, buf, 0)
+        torch.cuda.synchronize()
+        assert oracle.rel_l2(buf.cpu().numpy(), oracle.fft_dd(x, -1)) <= tol(n, bits)
+        buf.copy_(src)
+        torch.cuda.synchronize()
+        g.launch(0)
+        torch.cuda.synchronize()
+        assert torch.equal(buf, want)
+    finally:
+        p.destroy()
+    buf.copy_(src)
+    scratch = torch.randn(1 << 22, device="cuda")  # reuse freed memory
+    torch.cuda.synchronize()
+    g.launch(0)
+    torch.cuda.synchronize()
+    assert torch.equal(buf, want)
+    g.destroy()
+    del scratch

@@ -231,6 +231,139 @@ def test_coalesced_requests(gpu, hip_copy):
         assert oracle.rel_l2(mixed[tag], oracle.dft(x, d)) <= tol(n, 32), tag
 
 
+def test_poll_wait_returns_when_admission_cannot_progress(gpu, hip_copy):
+    """A held completion blocks the pending head's memory while nothing is
+    in flight: poll(wait=True) returns ALLOCATION_FAILURE instead of waiting
+    forever; after the release the head is admitted and completes."""
+    n, b = 1 << 16, 4
+    with gpu.Server(1 << 30, max_batch=4) as srv:
+        srv.submit(n, b, 64, -1, seed=1)
+        per = srv.stats().bytes_in_use
+        (c,) = srv.poll(wait=True)
+        srv.release(c.ticket)
+    with gpu.Server(per + per // 2, max_batch=4) as srv:
+        ta = srv.submit(n, b, 64, -1, seed=1, tag=1)
+        (ca,) = srv.poll(wait=True)
+        assert ca.ticket == ta and ca.status == gpu.SUCCESS
+        tb = srv.submit(n, b, 64, 1, seed=2, tag=2)   # needs A's memory
+        with pytest.raises(gpu.MofftError) as e:
+            srv.poll(wait=True)
+        assert e.value.status == gpu.ALLOCATION_FAILURE
+        srv.release(ta)
+        (cb,) = srv.poll(wait=True)
+        assert cb.ticket == tb and cb.status == gpu.SUCCESS
+        got = hip_copy(cb.output, n, b, 64)
+        srv.release(tb)
+    x = seeded_device_uniform(2, n * b, 64).reshape(b, n)
+    assert oracle.rel_l2(got, oracle.dft(x, 1)) <= tol(n, 64)
+
+
+def test_run_trace_fails_a_head_blocked_by_held_memory(gpu):
+    """run_trace with completions still held from before the call: a head
+    that cannot fit fails with ALLOCATION_FAILURE instead of stalling the
+    replay; the others complete."""
+    n, b = 1 << 16, 4
+    with gpu.Server(1 << 30, max_batch=4) as srv:
+        srv.submit(n, b, 64, -1, seed=1)
+        per = srv.stats().bytes_in_use
+        (c,) = srv.poll(wait=True)
+        srv.release(c.ticket)
+    with gpu.Server(per + per // 2, max_batch=4) as srv:
+        srv.submit(n, b, 64, -1, seed=1)
+        (held,) = srv.poll(wait=True)
+        reqs = [gpu.Request(n, b, 64, -1, None, 5, 0), gpu.Request(256, 1, 32, -1, None, 6, 1)]
+        cs = srv.run_trace(reqs, [0.0, 0.0])
+        assert cs[0].status == gpu.ALLOCATION_FAILURE
+        assert cs[1].status == gpu.SUCCESS
+
+
+def _copy_transform(ptr, n, t, bits):
+    """Transform t (n points) of a device result, to the host."""
+    rt = ctypes.CDLL("libamdhip64.so")
+    rt.hipMemcpy.restype = ctypes.c_int
+    rt.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    out = np.empty(n, np.complex64 if bits == 32 else np.complex128)
+    assert rt.hipMemcpy(out.ctypes.data, ptr + t * out.nbytes, out.nbytes, 2) == 0
+    return out
+
+
+def c5_requests(seed, count):
+    """Requests from the C5 serving distribution: N = 2^U{8..24}, batch
+    U{1..64}, 80 % FP32 / 20 % FP64, half forward, seeded device inputs."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for i in range(count):
+        n = 1 << int(rng.integers(8, 25))
+        b = int(rng.integers(1, 65))
+        bits = 32 if rng.random() < 0.8 else 64
+        d = -1 if rng.random() < 0.5 else 1
+        reqs.append((n, b, bits, d, int(rng.integers(1, 2**62)), i))
+    return reqs
+
+
+def test_c5_distribution_full_size(gpu, hip_copy):
+    """The C5 serving configuration at its stated sizes: 96 requests drawn
+    from its distribution plus its largest request, FP64 N = 2^24 x 64 (16
+    GiB in, 16 GiB out), submitted together under the 64 GB admission budget
+    so that admission binds (the large request waits for memory and holds
+    others back).  Every request completes successfully; for each length
+    N >= 2^21 one sampled transform of one request is checked against the
+    oracle at the precision bar; a subset of the shorter requests is served
+    again alone (one request at a time) and must be bit-identical to its
+    result in the mix."""
+    reqs = c5_requests(2000, 96)
+    big = (1 << 24, 64, 64, -1, 424242, len(reqs))
+    reqs.insert(10, big)
+    reqs = [(n, b, bits, d, seed, i) for i, (n, b, bits, d, seed, _) in enumerate(reqs)]
+    budget = 64 << 30
+    subset = [r for r in reqs if r[0] <= 1 << 16][:12]
+    sub_tags = {r[5] for r in subset}
+    checks = {}
+    for r in reqs:
+        if r[0] >= 1 << 21 and r[0] not in checks:
+            checks[r[0]] = r[5]
+    assert len(checks) == 4 and subset
+    mix, sampled = {}, {}
+    done = 0
+    with gpu.Server(budget, streams=4, max_inflight=16) as srv:
+        for (n, b, bits, d, seed, tag) in reqs:
+            srv.submit(n, b, bits, d, seed=seed, tag=tag)
+        st = srv.stats()
+        assert st.pending > 0          # admission binds: requests wait for memory
+        while done < len(reqs):
+            for c in srv.poll(wait=True):
+                assert c.status == gpu.SUCCESS, c.tag
+                n, b, bits = reqs[c.tag][:3]
+                if c.tag in sub_tags:
+                    mix[c.tag] = hip_copy(c.output, n, b, bits)
+                if c.tag in checks.values():
+                    t = int(np.random.default_rng(c.tag).integers(0, b))
+                    sampled[c.tag] = (t, _copy_transform(c.output, n, t, bits))
+                srv.release(c.ticket)
+                done += 1
+        st = srv.stats()
+        assert st.completed == len(reqs) and st.bytes_in_use == 0
+        assert st.peak_bytes_in_use <= budget
+        assert st.peak_bytes_in_use >= 2 * (1 << 24) * 64 * 16   # the big one ran
+    for n, tag in sorted(checks.items()):
+        _, b, bits, d, seed, _ = reqs[tag]
+        t, y = sampled[tag]
+        x = seeded_device_uniform(seed, n, bits, start=t * n).astype(np.complex128)
+        err = oracle.rel_l2(y, oracle.fft_dd(x[None, :], d)[0])
+        print(f"C5 check N=2^{n.bit_length() - 1} batch {b} FP{bits} transform {t}: rel-L2 {err:.2e}")
+        assert err <= tol(n, bits), (n, bits, err)
+    alone = {}
+    with gpu.Server(1 << 30, streams=1, max_inflight=1) as srv:
+        for (n, b, bits, d, seed, tag) in subset:
+            t = srv.submit(n, b, bits, d, seed=seed, tag=tag)
+            (c,) = srv.poll(wait=True)
+            assert c.ticket == t and c.status == gpu.SUCCESS
+            alone[tag] = hip_copy(c.output, n, b, bits)
+            srv.release(t)
+    for tag in sub_tags:
+        assert np.array_equal(alone[tag], mix[tag]), tag
+
+
 def test_coalescing_flag_rejects_unknown_bits(gpu):
     import ctypes as C
     o = gpu.ServerOptions()

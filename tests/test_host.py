@@ -465,3 +465,63 @@ def test_launch_element_bound():
     assert d.pass_count == 1 and d.chunk_transforms == (1 << 30) // 64
     d = mf.describe(1 << 12, 1 << 20, 64)
     assert d.chunk_transforms * (1 << 12) <= 1 << 30
+
+
+def _check_output(out, ref, bits, bound):
+    fn = ctypes.CDLL(mf.LIB_PATH).mofft_internal_check_output
+    fn.restype = ctypes.c_int
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                   ctypes.c_double]
+    return fn(out.ctypes.data, None if ref is None else ref.ctypes.data, out.size, bits, bound)
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+def test_measure_result_check(bits):
+    """MEASURE's result check (the reference rejects a candidate whose
+    output is not finite, src/mofft.c:725-737, 990-993; here a candidate
+    that disagrees with the analytical plan is also dropped), driven on host
+    buffers: finite and equal passes, a NaN or an infinity anywhere is
+    'not finite', a difference above the bound is 'disagree'."""
+    import numpy as np
+    dt = np.complex64 if bits == 32 else np.complex128
+    rng = np.random.default_rng(1)
+    ref = (rng.standard_normal(4096) + 1j * rng.standard_normal(4096)).astype(dt)
+    bound = 4e-6 if bits == 32 else 4e-15
+    assert _check_output(ref.copy(), ref, bits, bound) == 0
+    assert _check_output(ref.copy(), None, bits, bound) == 0
+    for bad in (np.nan, np.inf, -np.inf):
+        for where in (0, 1234, 4095):
+            y = ref.copy()
+            y[where] = complex(bad, 0) if where % 2 else complex(0, bad)
+            assert _check_output(y, ref, bits, bound) == 1
+            assert _check_output(y, None, bits, bound) == 1
+    y = ref.copy()
+    y[7] += 1e-2                      # rel-L2 ~1e-4: far above the bound
+    assert _check_output(y, ref, bits, bound) == 2
+    y = ref.copy()
+    y[7] *= 1 + (1e-7 if bits == 32 else 1e-16)
+    assert _check_output(y, ref, bits, bound) == 0
+
+
+@pytest.mark.parametrize("bits", [32, 64])
+@pytest.mark.parametrize("n,chain", [(1024, [4, 16, 16]), (1 << 16, [8, 8, 8, 8, 16]),
+                                     (30, [2, 15]), (20736, [12, 12, 12, 12]),
+                                     (64, [64]), (96, [32, 3])])
+def test_cpu_port_against_oracle(bits, n, chain):
+    """The timed CPU baseline (the reference executor with plan-time
+    tables, oracle/mofft_cpu_port.c) computes the DFT: forward against the
+    oracle and the round trip within the path's bars."""
+    import numpy as np
+    import oracle
+    dt = np.complex64 if bits == 32 else np.complex128
+    rng = np.random.default_rng(n)
+    x = (rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)).astype(dt)
+    y = np.empty_like(x)
+    z = np.empty_like(x)
+    oracle.PortPlan(n, chain, -1, bits).execute(x, y)
+    oracle.PortPlan(n, chain, 1, bits).execute(y, z)
+    bar = (1e-6 if bits == 32 else 1e-15) * math.log2(n)
+    assert oracle.rel_l2(y, oracle.dft(x.astype(np.complex128), -1)) <= bar
+    assert oracle.rel_l2(z / n, x) <= bar
+    with pytest.raises(ValueError):
+        oracle.PortPlan(n * 17, chain + [17], -1, bits)
