@@ -82,6 +82,32 @@ mofft_status mofft_execute_async_workspace_f64(
     mofft_complex_f64 *output, void *workspace, size_t workspace_bytes,
     void *stream);
 
+/* Allocation-free execute.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The execution of
+ * mofft_execute_async_workspace_* with the library's own per-device scratch
+ * array as the workspace: a static device array of MOFFT_SCRATCH_MAX_BYTES
+ * that each GPU the library runs on holds until the process ends (its
+ * address is resolved on a device's first call; no device allocation,
+ * free or synchronisation happens in this call).  Device buffers only.  A
+ * batch whose workspace exceeds the scratch runs in chunks that fit it; a
+ * plan whose single transform needs more workspace than the scratch (in
+ * place or three-pass plans of N x complex bytes > 64 MiB: FP64 N > 2^22,
+ * FP32 N > 2^23) is rejected with MOFFT_INVALID_ARGUMENT before anything
+ * runs.  Every call on one GPU shares that GPU's scratch, which keeps the
+ * last call's contents (nothing in it needs to start at zero): calls that
+ * overlap on one GPU - on different streams or from different host
+ * threads - race; order them on one stream. */
+#define MOFFT_SCRATCH_MAX_BYTES ((size_t)64 << 20)
+mofft_status mofft_execute_async_scratch_f32(const mofft_plan_f32 *plan,
+                                             const mofft_complex_f32 *input,
+                                             mofft_complex_f32 *output,
+                                             void *stream);
+mofft_status mofft_execute_async_scratch_f64(const mofft_plan_f64 *plan,
+                                             const mofft_complex_f64 *input,
+                                             mofft_complex_f64 *output,
+                                             void *stream);
+
 /* A plan's launches on fixed device buffers captured into a HIP graph: one
  * replay per execution, for launch-bound loops.  The graph holds a
  * reference to the plan's twiddle tables and workspace as they were at

@@ -11,9 +11,13 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <atomic>
+#include <mutex>
+
 #include "../mofft_abi.h"
 
 namespace mofft {
+constexpr int kMaxDevices = 64;
 typedef int (*launch_fn)(const mofft_pass_args *, void *);
 extern const launch_fn launch_table[];
 extern const int launch_table_size;
@@ -79,6 +83,12 @@ __global__ void fill_pass_kernel(T *out, int64_t ns, int64_t L) {
   out[2 * i] = (T)c;
   out[2 * i + 1] = (T)s;
 }
+/* Workspace of the allocation-free execute (mofft_execute_async_scratch_*):
+ * one array per device for the life of the process.  Host code never uses
+ * this name as an address: it is a host-side stand-in for the device
+ * symbol; mofft_hip_scratch resolves the device address once per device. */
+static __device__ __align__(256) unsigned char scratch[MOFFT_SCRATCH_BYTES];
+
 __device__ __forceinline__ uint64_t splitmix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -132,6 +142,34 @@ int mofft_hip_device_available(void) {
   if (hipGetDeviceProperties(&prop, device) != hipSuccess) { cached = 0; return 0; }
   cached = strncmp(prop.gcnArchName, "gfx950", 6) == 0 ? 1 : 0;
   return cached;
+}
+
+/* Device address of `scratch` on the current device: looked up with
+ * hipGetSymbolAddress the first time a device asks (that device current),
+ * then served from a per-device cache.  Each device holds its own copy of
+ * the array at its own address. */
+int mofft_hip_scratch(void **ptr, size_t *bytes) {
+  static std::mutex lock;
+  static std::atomic<void *> address[mofft::kMaxDevices];
+  if (!ptr || !bytes) return (int)hipErrorInvalidValue;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return (int)e;
+  if (dev < 0 || dev >= mofft::kMaxDevices) return (int)hipErrorInvalidDevice;
+  void *p = address[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> guard(lock);
+    p = address[dev].load(std::memory_order_relaxed);
+    if (!p) {
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(mofft::scratch));
+      if (e != hipSuccess) return (int)e;
+      if (!p) return (int)hipErrorInvalidSymbol;
+      address[dev].store(p, std::memory_order_release);
+    }
+  }
+  *ptr = p;
+  *bytes = MOFFT_SCRATCH_BYTES;
+  return 0;
 }
 
 int mofft_hip_malloc(void **ptr, size_t bytes) {

@@ -1538,6 +1538,52 @@ mofft_status mofft_execute_async_workspace_f64(
                        workspace_bytes, 0, stream);
 }
 
+/* The allocation-free execute: the plan's workspace replaced by the
+ * library's per-device scratch array (MOFFT_SCRATCH_BYTES), chunks shrunk
+ * to what it holds.  No device allocation and no synchronisation. */
+_Static_assert(MOFFT_SCRATCH_BYTES == MOFFT_SCRATCH_MAX_BYTES,
+               "one scratch size in the ABI and the public header");
+
+static mofft_status execute_scratch(const mofft_plan_common *p, const void *in,
+                                    void *out, void *stream) {
+  if (!p || !in || !out) return MOFFT_INVALID_ARGUMENT;
+  if ((p->placement == MOFFT_IN_PLACE) != (in == out))
+    return MOFFT_INVALID_ARGUMENT;
+  if (!mofft_hip_is_device_pointer(in) || !mofft_hip_is_device_pointer(out))
+    return MOFFT_INVALID_ARGUMENT;
+  const size_t need = schedule_workspace(&p->s, p->placement);
+  if (need == 0)
+    return launch_schedule(p, in, out, NULL, stream) == 0 ? MOFFT_SUCCESS
+                                                          : MOFFT_INTERNAL_ERROR;
+  /* workspace per transform: one transform's intermediate */
+  const size_t per = need / p->s.chunk;
+  void *scratch = NULL;
+  size_t scratch_bytes = 0;
+  if (mofft_hip_scratch(&scratch, &scratch_bytes) != 0 || !scratch)
+    return MOFFT_INTERNAL_ERROR;
+  if (per == 0 || per > scratch_bytes) return MOFFT_INVALID_ARGUMENT;
+  size_t chunk = scratch_bytes / per;
+  if (chunk > p->s.chunk) chunk = p->s.chunk;
+  return launch_schedule_chunks(p, in, out, scratch, stream, p->s.batch, chunk,
+                                NULL) == 0
+             ? MOFFT_SUCCESS
+             : MOFFT_INTERNAL_ERROR;
+}
+
+mofft_status mofft_execute_async_scratch_f32(const mofft_plan_f32 *plan,
+                                             const mofft_complex_f32 *input,
+                                             mofft_complex_f32 *output,
+                                             void *stream) {
+  return execute_scratch(plan ? &plan->c : NULL, input, output, stream);
+}
+
+mofft_status mofft_execute_async_scratch_f64(const mofft_plan_f64 *plan,
+                                             const mofft_complex_f64 *input,
+                                             mofft_complex_f64 *output,
+                                             void *stream) {
+  return execute_scratch(plan ? &plan->c : NULL, input, output, stream);
+}
+
 mofft_status mofft_plan_get_batch_f32(const mofft_plan_f32 *plan,
                                       size_t *batch) {
   if (!plan || !batch) return MOFFT_INVALID_ARGUMENT;

@@ -112,6 +112,11 @@ int mofft_hip_copy(void *dst, const void *src, size_t bytes);
 int mofft_hip_copy_async(void *dst, const void *src, size_t bytes,
                          void *stream);
 int mofft_hip_synchronize(void *stream);
+/* Bytes of the per-device scratch array behind the allocation-free
+ * execute, and its device address on the current device (resolved once
+ * per device; no allocation). */
+#define MOFFT_SCRATCH_BYTES ((size_t)64 << 20)
+int mofft_hip_scratch(void **ptr, size_t *bytes);
 /* out[i] = w_M^((i * step) mod M) for i < count, rounded once from FP64. */
 int mofft_hip_fill_powers(void *out, int precision_bits, int64_t count,
                           int64_t M, int64_t step, void *stream);
