@@ -174,6 +174,24 @@ mofft_status mofft_plan_get_description_f32(const mofft_plan_f32 *plan,
 mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
                                             mofft_plan_description *out);
 
+/* The analytical planner's cost model (MI355X counterpart of the
+ * reference's generated cost functions, compiler/mofft_compiler/
+ * emitter.py:621-725): predicted device seconds of `batch` transforms under
+ * the schedule a chain, pass mask and kernel bank give - every pass moves
+ * the batch once through HBM at its kernel's measured cost in its role
+ * (single pass, first strided pass, strided pass with twiddles; the table
+ * `mofft_tune --pass-costs` writes, compiled in), plus the generated
+ * codelets' arithmetic at the VALU peak.  DBL_MAX for an invalid plan.  The
+ * PRETUNED model plan (lengths without wisdom) and MEASURE's candidate
+ * order are the lowest predictions. */
+double mofft_estimated_schedule_seconds(size_t length, size_t batch,
+                                        int precision_bits, const int *radices,
+                                        size_t radix_count, uint64_t pass_mask,
+                                        unsigned int kernel_bank);
+/* 1 when the compiled pass-cost table holds measurements (else the planner
+ * uses nominal pass weights). */
+int mofft_estimated_pass_costs_measured(void);
+
 /* Kernel registry introspection (host only). */
 typedef struct {
   int precision_bits;

@@ -233,6 +233,9 @@ _SIGS += [
     ("mofft_version", ctypes.c_char_p, []),
     ("mofft_plan_describe", _I,
      [_SZ, _SZ, _I, ctypes.POINTER(_I), _SZ, _U64, ctypes.POINTER(PlanDescription)]),
+    ("mofft_estimated_schedule_seconds", ctypes.c_double,
+     [_SZ, _SZ, _I, ctypes.POINTER(_I), _SZ, _U64, ctypes.c_uint]),
+    ("mofft_estimated_pass_costs_measured", _I, []),
     ("mofft_kernel_count", _I, []),
     ("mofft_kernel_get_info", _I, [_I, ctypes.POINTER(KernelInfo)]),
     ("mofft_device_available", _I, []),
@@ -640,6 +643,16 @@ def server_generate(out, precision_bits: int, count: int, seed: int, stream: int
     """The server's seeded device input generator on a caller buffer."""
     _check(lib().mofft_server_generate(_ptr(out), precision_bits, count, seed, stream or None),
            "server_generate")
+
+
+def estimated_schedule_seconds(length: int, batch: int, precision_bits: int, radices,
+                               mask: int = 0, bank: int = 0) -> float:
+    """The planner's predicted device seconds of a schedule
+    (mofft_estimated_schedule_seconds); inf for an invalid plan."""
+    arr, n = _int_array(radices)
+    v = lib().mofft_estimated_schedule_seconds(length, batch, precision_bits, arr, n,
+                                               mask, bank)
+    return float("inf") if v > 1e300 else v
 
 
 def estimated_plan_breakdown(length: int, precision_bits: int, radices,

@@ -27,6 +27,7 @@ BUILD = PKG / "_build"
 LIB_DIR = PKG / "lib"
 LIB = LIB_DIR / "libmofft.so"
 WISDOM_JSON = PKG / "wisdom" / "gfx950.json"
+PASSES_JSON = PKG / "wisdom" / "gfx950_passes.json"   # mofft_tune --pass-costs
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CC = os.environ.get("CC", "gcc")
@@ -41,7 +42,7 @@ C_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-Wall", "-Wextra",
 def _generate() -> None:
     sys.path.insert(0, str(PKG / "compiler"))
     from mofft_compiler import emit, wisdom  # noqa: E402
-    emit.emit(GEN)
+    emit.emit(GEN, passes_json=PASSES_JSON if PASSES_JSON.exists() else None)
     text = wisdom.emit_c(WISDOM_JSON)
     path = GEN / "mofft_wisdom.c"
     if not path.exists() or path.read_text() != text:
@@ -52,6 +53,7 @@ def _sources() -> tuple[list[Path], list[Path], list[Path]]:
     c_srcs = [CSRC / "mofft.c", CSRC / "mofft_batch.c", CSRC / "mofft_server.cpp",
               GEN / "mofft_registry.c",
               GEN / "mofft_wisdom.c",
+              GEN / "mofft_costs.c",
               GEN / "mofft_launch_table.cpp"]
     hip_srcs = [CSRC / "hip" / "device.hip"]
     hip_srcs += sorted(GEN.glob("mofft_kernels_p*.hip"))
@@ -64,7 +66,8 @@ def _sources() -> tuple[list[Path], list[Path], list[Path]]:
 def build_id() -> str:
     c_srcs, hip_srcs, headers = _sources()
     h = hashlib.sha256()
-    for p in sorted(set(c_srcs + hip_srcs + headers + [WISDOM_JSON])):
+    extra = [WISDOM_JSON] + ([PASSES_JSON] if PASSES_JSON.exists() else [])
+    for p in sorted(set(c_srcs + hip_srcs + headers + extra)):
         h.update(str(p.relative_to(ROOT)).encode())
         h.update(b"\0")
         h.update(p.read_bytes())

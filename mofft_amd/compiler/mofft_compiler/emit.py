@@ -158,6 +158,69 @@ def emit_registry_c(es: list[registry.Entry], bs: list[registry.Entry]) -> str:
     return "\n".join(lines)
 
 
+PASS_ROLES = {"rows": 0, "first": 1, "later": 2}
+PASS_SPANS = (16, 64, 256, 1024)   # strides / spans the strided roles are timed at
+
+
+def codelet_flops(counts: dict) -> dict[int, int]:
+    """Real flops of one radix-r DFT codelet (adds + multiplies of the
+    generated straight-line code; negations are free), and of the radix
+    32 / 64 levels, which run as two-stage L = 32 (8 x 4) / 64 (8 x 8)
+    kernels: their codelets plus one complex twiddle product (6 flops) per
+    non-trivial point of the second stage."""
+    out = {r: c["add"] + c["mul"] for r, c in counts.items()}
+    for L, (a, b) in ((32, (8, 4)), (64, (8, 8))):
+        out[L] = (L // a) * out[a] + (L // b) * out[b] + 6 * (L - L // b)
+    return out
+
+
+def emit_costs_c(es: list[registry.Entry], counts: dict,
+                 passes_json: Path | None) -> str:
+    """The analytical planner's measured data: generated codelet flop
+    counts by radix, and the pass-cost table (ns per byte moved) of every
+    registry entry in each role it was timed in, from `mofft_tune
+    --pass-costs` output: column 0 as the single pass of N = L (rows),
+    columns 1-4 as a strided first pass at stride 16 / 64 / 256 / 1024,
+    columns 5-8 as a strided pass with pass twiddles at span 16 / 64 / 256
+    / 1024; 0 = not measured."""
+    flops = codelet_flops(counts)
+    index = {(e.precision, e.rows, e.length, tuple(e.stages), e.variant): i
+             for i, e in enumerate(es)}
+    ns = [[0.0] * 9 for _ in es]
+    source = "none"
+    if passes_json is not None and Path(passes_json).exists():
+        payload = json.loads(Path(passes_json).read_text())
+        if payload.get("schema_version") != 2 or payload.get("target") != "gfx950":
+            raise ValueError("pass-cost table of another schema or target")
+        source = str(payload.get("source", "none"))
+        for e in payload["entries"]:
+            key = (int(e["precision"]), int(e["rows"]), int(e["length"]),
+                   tuple(int(r) for r in e["stages"]), int(e["variant"]))
+            i = index.get(key)
+            if i is None:
+                continue   # a kernel no longer compiled
+            if e["bytes"] <= 0 or e["seconds"] <= 0:
+                raise ValueError(f"pass cost must be positive: {e}")
+            role = PASS_ROLES[e["role"]]
+            col = 0 if role == 0 else (1 if role == 1 else 5) + PASS_SPANS.index(int(e["span"]))
+            ns[i][col] = 1e9 * float(e["seconds"]) / float(e["bytes"])
+    lines = [HEADER, '#include "../mofft_internal.h"', "",
+             "/* flops of one radix-r codelet, index r (0: no codelet) */",
+             "const double mofft_codelet_flops[65] = {"]
+    lines.append("  " + ", ".join(f"{float(flops.get(r, 0))}" for r in range(65)))
+    lines += ["};", "",
+              "/* ns per byte moved (one read + one write of the data) of every",
+              "   registry entry: [rows, first at stride 16, 64, 256, 1024,",
+              "   later at span 16, 64, 256, 1024]; 0 = not measured */",
+              f"const double mofft_pass_cost_ns[{max(1, len(es))}][9] = {{"]
+    for i, row in enumerate(ns):
+        lines.append("  {" + ", ".join(f"{v:.5e}" for v in row) + f"}},  /* {es[i].symbol} */")
+    if not es:
+        lines.append("  {0.0},")
+    lines += ["};", f"const char *const mofft_pass_cost_source = {json.dumps(source)};", ""]
+    return "\n".join(lines)
+
+
 def emit_batch_part(bs: list[registry.Entry], precision: int) -> str:
     """One precision's persistent batch kernel: a switch over its kinds."""
     t = "float" if precision == 32 else "double"
@@ -218,7 +281,8 @@ def emit_launch_table(es: list[registry.Entry]) -> str:
     return "\n".join(lines)
 
 
-def emit(out_dir: str | Path, parts: int = 24) -> dict:
+def emit(out_dir: str | Path, parts: int = 24,
+         passes_json: str | Path | None = None) -> dict:
     out = Path(out_dir)
     out.mkdir(parents=True, exist_ok=True)
     es = registry.entries()
@@ -238,6 +302,7 @@ def emit(out_dir: str | Path, parts: int = 24) -> dict:
     for k, idx in enumerate(buckets):
         files[f"mofft_kernels_p{k}.hip"] = emit_kernel_part(es, sorted(idx))
     files["mofft_launch_table.cpp"] = emit_launch_table(es)
+    files["mofft_costs.c"] = emit_costs_c(es, counts, Path(passes_json) if passes_json else None)
     digest = hashlib.sha256()
     for name in sorted(files):
         digest.update(name.encode())

@@ -23,6 +23,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -401,7 +402,8 @@ static size_t schedule_workspace(const mofft_schedule *s,
 
 typedef struct {
   int count;
-  int weight; /* sum of pass_weight, tenths of a pass */
+  long long weight; /* sum of pass weights (pass_weight tenths, or measured
+                       ns per byte in millionths) */
   size_t lengths[MOFFT_MAX_PLAN_DEPTH];
   int kernels[MOFFT_MAX_PLAN_DEPTH];
 } pass_split;
@@ -422,6 +424,84 @@ static int pass_weight(const mofft_registry_entry *e) {
   return w; /* tenths: integer sums compare exactly */
 }
 
+static double codelet_flops(int radix);
+
+/* Measured pass cost of registry entry `kernel` in a role (MOFFT_ROLE_*)
+ * at a stride (first pass) or span (later pass) `span`, in ns per byte
+ * moved, from the pass-cost table `mofft_tune --pass-costs` writes (each
+ * kernel timed as the pass it is in plans, 2^24 FP32 / 2^23 FP64 points,
+ * strided ones at spans 16, 64, 256, 1024): linear in log2 span between
+ * the measured spans, the nearest one outside them; 0 when the table has
+ * no entry for it. */
+static double measured_pass_ns(int kernel, int role, size_t span) {
+  if (kernel < 0 || kernel >= mofft_registry_count) return 0.0;
+  const double *row = mofft_pass_cost_ns[kernel];
+  if (role == MOFFT_ROLE_ROWS) return row[0];
+  const double *v = row + (role == MOFFT_ROLE_FIRST ? 1 : 5);
+  const double x = log2((double)(span < 1 ? 1 : span));
+  static const double at[4] = {4.0, 6.0, 8.0, 10.0};   /* log2 of 16 .. 1024 */
+  double lo_x = 0, lo_v = 0, hi_x = 0, hi_v = 0;
+  int have_lo = 0, have_hi = 0;
+  for (int q = 0; q < 4; ++q) {
+    if (v[q] <= 0) continue;
+    if (at[q] <= x && (!have_lo || at[q] > lo_x)) { lo_x = at[q]; lo_v = v[q]; have_lo = 1; }
+    if (at[q] >= x && (!have_hi || at[q] < hi_x)) { hi_x = at[q]; hi_v = v[q]; have_hi = 1; }
+  }
+  if (have_lo && have_hi)
+    return hi_x > lo_x ? lo_v + (hi_v - lo_v) * (x - lo_x) / (hi_x - lo_x) : lo_v;
+  if (have_lo) return lo_v;
+  if (have_hi) return hi_v;
+  return 0.0;
+}
+
+/* Nominal cost of a pass the table does not cover: the measured pass
+ * weights below at 5.5 TB/s (ns per byte). */
+static double nominal_pass_ns(const mofft_registry_entry *e) {
+  return (e->rows ? 10.0 : (double)pass_weight(e)) / 10.0 / 5500.0;
+}
+
+static double pass_ns_per_byte(int kernel, int role, size_t span) {
+  const double m = measured_pass_ns(kernel, role, span);
+  return m > 0 ? m : nominal_pass_ns(&mofft_registry[kernel]);
+}
+
+/* Role of pass i of a schedule over the pass kernels, and its stride
+ * (first pass) or span (later passes). */
+static int pass_role(const mofft_schedule *s, int i, size_t *span) {
+  *span = i == 0 ? s->passes[i].stride : s->passes[i].span;
+  if (mofft_registry[s->passes[i].kernel].rows) return MOFFT_ROLE_ROWS;
+  return i == 0 ? MOFFT_ROLE_FIRST : MOFFT_ROLE_LATER;
+}
+
+/* Predicted device seconds of a schedule over the pass kernels: every pass
+ * moves the batch once (read + write) at its kernel's measured cost in its
+ * role, plus the codelet arithmetic of its stages at the VALU peak (a small
+ * term; it separates plans whose passes cost the same). */
+static double schedule_seconds(const mofft_schedule *s) {
+  const double bytes = 2.0 * (double)s->length * (double)s->batch *
+                       (double)complex_bytes(s->precision_bits);
+  double ns = 0.0;
+  for (int i = 0; i < s->pass_count; ++i) {
+    size_t span = 1;
+    const int role = pass_role(s, i, &span);
+    ns += bytes * pass_ns_per_byte(s->passes[i].kernel, role, span);
+  }
+  double flops = 0.0;
+  for (int l = 0; l < s->depth; ++l) {
+    const double f = codelet_flops(s->factors[l]);
+    if (f > 0) flops += f * (double)(s->length / (size_t)s->factors[l]);
+  }
+  const double peak = s->precision_bits == 32 ? 157.3e3 : 78.6e3; /* flop/ns */
+  ns += flops * (double)s->batch / peak;
+  return ns * 1e-9;
+}
+
+/* Integer weight of a pass for the split search: measured ns per byte in
+ * millionths (sums compare exactly). */
+static long long pass_weight_measured(int kernel, int role, size_t span) {
+  return llround(pass_ns_per_byte(kernel, role, span) * 1e6);
+}
+
 /* better = lower HBM cost, then fewer passes, then larger smallest pass,
  * then larger first pass */
 static int split_better(const pass_split *a, const pass_split *b) {
@@ -437,65 +517,168 @@ static int split_better(const pass_split *a, const pass_split *b) {
   return 0;
 }
 
+/* 1 when the compiled pass-cost table holds measurements and
+ * MOFFT_PASS_COSTS=0 in the environment does not switch it off (a tuning
+ * knob for comparing against the nominal weights; read once per process) */
+static int pass_costs_measured(void) {
+  static int cached = -1;
+  if (cached < 0) {
+    const char *env = getenv("MOFFT_PASS_COSTS");
+    cached = strcmp(mofft_pass_cost_source, "none") != 0 &&
+             !(env && strcmp(env, "0") == 0);
+  }
+  return cached;
+}
+
+/* Weight of pass-kernel entry i at position `count` of a split, at
+ * stride (first pass) or span (later) `span`: the measured cost when the
+ * table is present, else the nominal tenths. */
+static long long split_weight(const mofft_kernel_set *ks, int i, int count,
+                              size_t span, int measured) {
+  if (ks != PASS_KERNELS) return 10; /* batch kinds: passes of equal cost */
+  if (measured)
+    return pass_weight_measured(i, count == 0 ? MOFFT_ROLE_FIRST : MOFFT_ROLE_LATER,
+                                span);
+  return pass_weight(&ks->entries[i]);
+}
+
+/* the cheapest any pass can be (prunes the split search) */
+static long long min_split_weight(const mofft_kernel_set *ks, int precision_bits,
+                                  int measured) {
+  long long m = -1;
+  static const size_t spans[] = {16, 64, 256, 1024};
+  for (int i = 0; i < *ks->count; ++i) {
+    const mofft_registry_entry *e = &ks->entries[i];
+    if (e->precision_bits != precision_bits || e->rows || e->variant != 0) continue;
+    for (int count = 0; count < 2; ++count)
+      for (int q = 0; q < 4; ++q) {
+        const long long w = split_weight(ks, i, count, spans[q], measured);
+        if (m < 0 || w < m) m = w;
+      }
+  }
+  return m < 0 ? 10 : m;
+}
+
+/* Distinct strided pass lengths of a kernel set and precision, in
+ * registry order of their first bank-0 kernel (built once per set). */
+typedef struct {
+  int count;
+  size_t length[512];
+  int first[512];       /* first bank-0 strided kernel of that length */
+  int nk[512];          /* bank-0 strided kernels of that length (stage chains) */
+  int kernels[512][8];
+} split_lengths;
+
+static const split_lengths *strided_lengths(const mofft_kernel_set *ks,
+                                            int precision_bits) {
+  static split_lengths table[2][2];      /* [pass / batch set][FP32 / FP64] */
+  static int built[2][2];
+  static pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+  const int a = ks == PASS_KERNELS ? 0 : 1, b = precision_bits == 32 ? 0 : 1;
+  pthread_mutex_lock(&lock);
+  if (!built[a][b]) {
+    split_lengths *t = &table[a][b];
+    t->count = 0;
+    for (int i = 0; i < *ks->count && t->count < 512; ++i) {
+      const mofft_registry_entry *e = &ks->entries[i];
+      if (e->precision_bits != precision_bits || e->rows || e->variant != 0 ||
+          e->length < 2)
+        continue;
+      int seen = -1;
+      for (int j = 0; j < t->count && seen < 0; ++j)
+        if (t->length[j] == (size_t)e->length) seen = j;
+      if (seen >= 0) {
+        if (t->nk[seen] < 8) t->kernels[seen][t->nk[seen]++] = i;
+        continue;
+      }
+      t->length[t->count] = (size_t)e->length;
+      t->first[t->count] = i;
+      t->nk[t->count] = 1;
+      t->kernels[t->count][0] = i;
+      t->count++;
+    }
+    built[a][b] = 1;
+  }
+  pthread_mutex_unlock(&lock);
+  return &table[a][b];
+}
+
 static void search_splits(const mofft_kernel_set *ks, int precision_bits,
                           size_t remaining, pass_split *cur, pass_split *best,
-                          int max_passes) {
+                          int max_passes, long long min_w, int measured) {
+  /* product of the passes chosen so far: the next pass's span */
+  size_t done = 1;
+  for (int i = 0; i < cur->count; ++i) done *= cur->lengths[i];
   if (remaining == 1) {
     if (cur->count > 0 && split_better(cur, best)) *best = *cur;
     return;
   }
   if (cur->count >= max_passes) return;
-  /* every further pass costs at least 1 */
-  if (best->count != 0 && cur->weight + 10 > best->weight) return;
-  for (int i = 0; i < *ks->count; ++i) {
-    const mofft_registry_entry *e = &ks->entries[i];
-    if (e->precision_bits != precision_bits || e->rows || e->variant != 0)
-      continue;
-    const size_t l = (size_t)e->length;
-    if (l < 2 || remaining % l) continue;
-    int dup = 0;
-    for (int j = 0; j < i; ++j) {
-      const mofft_registry_entry *f = &ks->entries[j];
-      if (f->precision_bits == precision_bits && !f->rows &&
-          f->length == e->length) { dup = 1; break; }
+  /* every further pass costs at least min_w */
+  if (best->count != 0 && cur->weight + min_w > best->weight) return;
+  const split_lengths *sl = strided_lengths(ks, precision_bits);
+  for (int q = 0; q < sl->count; ++q) {
+    const size_t l = sl->length[q];
+    if (remaining % l) continue;
+    /* one candidate per pass length: its chosen kernel; a first pass runs
+     * at stride N / l, a later one at span `done` */
+    const size_t span = cur->count == 0 ? remaining / l : done;
+    int k = sl->first[q];
+    if (measured && ks == PASS_KERNELS) {
+      long long bw = split_weight(ks, k, cur->count, span, measured);
+      for (int j = 1; j < sl->nk[q]; ++j) {
+        const long long w = split_weight(ks, sl->kernels[q][j], cur->count, span, measured);
+        if (w < bw) { bw = w; k = sl->kernels[q][j]; }
+      }
     }
-    if (dup) continue;
-    /* the weights are pass-kernel timings; the persistent batch's
-     * 256-thread kinds were measured as passes of equal cost */
-    const int w = ks == PASS_KERNELS ? pass_weight(e) : 10;
+    const long long w = split_weight(ks, k, cur->count, span, measured);
     cur->lengths[cur->count] = l;
-    cur->kernels[cur->count] = i;
+    cur->kernels[cur->count] = k;
     cur->count++;
     cur->weight += w;
-    search_splits(ks, precision_bits, remaining / l, cur, best, max_passes);
+    search_splits(ks, precision_bits, remaining / l, cur, best, max_passes, min_w,
+                  measured);
     cur->weight -= w;
     cur->count--;
   }
 }
 
-/* Model plan: a single-pass kernel when one covers the length, otherwise the
- * pass decomposition with the fewest HBM round trips. */
-static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
-                                int precision_bits, int *radices,
-                                size_t *depth, uint64_t *mask) {
-  if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
-  const int single = find_kernel_by_length(ks, precision_bits, 1, length);
-  if (single >= 0) {
-    const mofft_registry_entry *e = &ks->entries[single];
-    for (int i = 0; i < e->stage_count; ++i)
-      radices[i] = e->stages[e->stage_count - 1 - i];
-    *depth = (size_t)e->stage_count;
-    *mask = 0;
-    return MOFFT_SUCCESS;
+/* The single-pass kernel of a length: with measured costs the cheapest
+ * whole-transform kernel of any stage chain (tile banks 0, 1, 2, 4), else
+ * the first bank-0 one compiled. */
+static int single_pass_kernel(const mofft_kernel_set *ks, int precision_bits,
+                              size_t length) {
+  if (ks != PASS_KERNELS || !pass_costs_measured())
+    return find_kernel_by_length(ks, precision_bits, 1, length);
+  int best = -1;
+  double bn = 0.0;
+  for (int i = 0; i < *ks->count; ++i) {
+    const mofft_registry_entry *e = &ks->entries[i];
+    if (e->precision_bits != precision_bits || !e->rows ||
+        (size_t)e->length != length || e->variant == (int)MOFFT_MFMA_BANK)
+      continue;
+    const double n = pass_ns_per_byte(i, MOFFT_ROLE_ROWS, 1);
+    if (best < 0 || n < bn) { best = i; bn = n; }
   }
+  return best;
+}
+
+/* The multi-pass split of the lowest weight (measured or nominal), as a
+ * chain (outer -> inner) and pass mask. */
+static mofft_status best_split(const mofft_kernel_set *ks, size_t length,
+                               int precision_bits, int measured, int *radices,
+                               size_t *depth, uint64_t *mask) {
   pass_split cur, best;
   memset(&cur, 0, sizeof(cur));
   memset(&best, 0, sizeof(best));
-  /* a plan of more than max_passes passes costs more than max_passes, so
-   * the search stops once the best plan found costs no more than that */
+  const long long min_w = min_split_weight(ks, precision_bits, measured);
+  /* a plan of more than max_passes passes costs at least
+   * min_w * (max_passes + 1), so the search stops once the best plan found
+   * costs no more than that */
   for (int max_passes = 2; max_passes <= 12; max_passes += 2) {
-    search_splits(ks, precision_bits, length, &cur, &best, max_passes);
-    if (best.count != 0 && best.weight <= 10 * max_passes) break;
+    search_splits(ks, precision_bits, length, &cur, &best, max_passes, min_w,
+                  measured);
+    if (best.count != 0 && best.weight <= min_w * (max_passes + 1)) break;
   }
   if (best.count == 0) return MOFFT_UNSUPPORTED_SIZE;
   /* concatenate: the last pass is outermost */
@@ -511,6 +694,78 @@ static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
   *depth = d;
   *mask = m;
   return MOFFT_SUCCESS;
+}
+
+static double chain_seconds(size_t length, int precision_bits, const int *radices,
+                            size_t depth, uint64_t mask) {
+  mofft_schedule s;
+  if (build_schedule(PASS_KERNELS, length, 1, precision_bits, radices, depth,
+                     mask, 0, &s) != MOFFT_SUCCESS)
+    return DBL_MAX;
+  return schedule_seconds(&s);
+}
+
+/* A multi-pass split the measured model prefers is taken over the split of
+ * the nominal pass weights only when it is predicted this much faster: the
+ * per-pass measurements (each kernel alone at four strides / spans) miss
+ * effects of the actual neighbours, and on 58 untuned lengths (2^24
+ * points, MI355X) taking every measured-model split ran 3 % faster in the
+ * geometric mean but 17 lengths more than 5 % slower (up to 24 %); with
+ * this margin 5 % faster, 2 lengths more than 5 % slower (at most 9 %). */
+#define MOFFT_MODEL_SPLIT_MARGIN 0.85
+
+/* Model plan: a single-pass kernel when one covers the length (with
+ * measured costs the cheapest of its stage chains), otherwise the pass
+ * decomposition of the lowest predicted HBM cost. */
+static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
+                                int precision_bits, int *radices,
+                                size_t *depth, uint64_t *mask) {
+  if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
+  const int single = single_pass_kernel(ks, precision_bits, length);
+  if (single >= 0) {
+    const mofft_registry_entry *e = &ks->entries[single];
+    for (int i = 0; i < e->stage_count; ++i)
+      radices[i] = e->stages[e->stage_count - 1 - i];
+    *depth = (size_t)e->stage_count;
+    *mask = 0;
+    return MOFFT_SUCCESS;
+  }
+  mofft_status st = best_split(ks, length, precision_bits, 0, radices, depth, mask);
+  if (st != MOFFT_SUCCESS || ks != PASS_KERNELS || !pass_costs_measured())
+    return st;
+  int r2[MOFFT_MAX_PLAN_DEPTH];
+  size_t d2 = 0;
+  uint64_t m2 = 0;
+  if (best_split(ks, length, precision_bits, 1, r2, &d2, &m2) != MOFFT_SUCCESS)
+    return st;
+  const double nominal = chain_seconds(length, precision_bits, radices, *depth, *mask);
+  const double measured = chain_seconds(length, precision_bits, r2, d2, m2);
+  if (measured < MOFFT_MODEL_SPLIT_MARGIN * nominal) {
+    memcpy(radices, r2, d2 * sizeof(int));
+    *depth = d2;
+    *mask = m2;
+  }
+  return MOFFT_SUCCESS;
+}
+
+/* The tile bank of the model plan: with measured costs, the plain bank
+ * (0, 1, 2, 4) whose schedule of the chain is predicted fastest; else 0. */
+static unsigned int model_bank(size_t length, int precision_bits,
+                               const int *radices, size_t depth,
+                               uint64_t mask) {
+  if (!pass_costs_measured()) return 0;
+  static const unsigned int banks[] = {0u, 1u, 2u, MOFFT_WIDE_BANK};
+  unsigned int best = 0;
+  double bt = 0.0;
+  for (size_t i = 0; i < sizeof(banks) / sizeof(banks[0]); ++i) {
+    mofft_schedule s;
+    if (build_schedule(PASS_KERNELS, length, 1, precision_bits, radices, depth,
+                       mask, banks[i], &s) != MOFFT_SUCCESS)
+      continue;
+    const double t = schedule_seconds(&s);
+    if (i == 0 || t < bt) { best = banks[i]; bt = t; }
+  }
+  return best;
 }
 
 /* ------------------------------------------------------------------ */
@@ -899,6 +1154,8 @@ static double scalar_median(double *v, size_t n) {
 }
 
 static double schedule_model_cost(const mofft_schedule *s) {
+  /* the measured pass costs when the table is present */
+  if (pass_costs_measured()) return schedule_seconds(s);
   /* HBM round trips dominate: one read + one write per pass, each strided
    * pass weighted by its measured cost (pass_weight), plus a small
    * per-stage compute term */
@@ -948,7 +1205,8 @@ static void add_candidate(measured_candidate *pool, size_t *count, size_t cap,
     c.depth = depth;
     c.mask = s.pass_mask; /* effective grouping */
     c.bank = bank;
-    c.model = schedule_model_cost(&s) * (1.0 + 0.01 * bank);
+    c.model = pass_costs_measured() ? schedule_model_cost(&s)
+                                    : schedule_model_cost(&s) * (1.0 + 0.01 * bank);
     int dup = 0;
     for (size_t i = 0; i < *count; ++i)
       if (same_candidate(&pool[i], &c)) { dup = 1; break; }
@@ -1272,9 +1530,10 @@ static mofft_status create_with_options(mofft_plan_common **out,
     if (options.flags & MOFFT_PLAN_STRICT_WISDOM) return MOFFT_WISDOM_NOT_FOUND;
     st = model_chain(PASS_KERNELS, length, precision_bits, radices, &depth, &mask);
     if (st != MOFFT_SUCCESS) return st;
+    bank = model_bank(length, precision_bits, radices, depth, mask);
   }
   st = plan_from_chain(out, length, batch, precision_bits, direction,
-                       placement, radices, depth, mask, w > 0 ? bank : 0);
+                       placement, radices, depth, mask, bank);
   if (st != MOFFT_SUCCESS) return st;
   if (w > 0) {
     (*out)->direct_input = direct;
@@ -1642,11 +1901,11 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
   int chain[MOFFT_MAX_PLAN_DEPTH];
   size_t depth = radix_count;
   uint64_t mask = pass_mask;
+  unsigned int bank = 0;
   if (!radices) {
     if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
     size_t d2 = 0;
     int direct = 0, layout = 0;
-    unsigned int bank = 0;
     const int w = wisdom_lookup(length, precision_bits, chain, &d2, &mask,
                                 &direct, &layout, &bank);
     if (w < 0) return MOFFT_WISDOM_NOT_FOUND;
@@ -1654,6 +1913,7 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
       mofft_status st = model_chain(PASS_KERNELS, length, precision_bits, chain,
                                     &d2, &mask);
       if (st != MOFFT_SUCCESS) return st;
+      bank = model_bank(length, precision_bits, chain, d2, mask);
     }
     depth = d2;
   } else {
@@ -1663,11 +1923,28 @@ mofft_status mofft_plan_describe(size_t length, size_t batch,
   }
   mofft_schedule s;
   mofft_status st = build_schedule(PASS_KERNELS, length, batch, precision_bits,
-                                   chain, depth, mask, 0, &s);
+                                   chain, depth, mask, bank, &s);
   if (st != MOFFT_SUCCESS) return st;
   describe_schedule(&s, MOFFT_IN_PLACE, description);
   return MOFFT_SUCCESS;
 }
+
+double mofft_estimated_schedule_seconds(size_t length, size_t batch,
+                                        int precision_bits, const int *radices,
+                                        size_t radix_count, uint64_t pass_mask,
+                                        unsigned int kernel_bank) {
+  if (!radices || radix_count == 0 || radix_count > MOFFT_MAX_PLAN_DEPTH ||
+      batch == 0 || (precision_bits != 32 && precision_bits != 64) ||
+      kernel_bank >= kernel_bank_count())
+    return DBL_MAX;
+  mofft_schedule s;
+  if (build_schedule(PASS_KERNELS, length, batch, precision_bits, radices,
+                     radix_count, pass_mask, kernel_bank, &s) != MOFFT_SUCCESS)
+    return DBL_MAX;
+  return schedule_seconds(&s);
+}
+
+int mofft_estimated_pass_costs_measured(void) { return pass_costs_measured(); }
 
 mofft_status mofft_plan_get_description_f32(const mofft_plan_f32 *plan,
                                             mofft_plan_description *out) {
@@ -2026,15 +2303,12 @@ mofft_status mofft_plan_import_f64(mofft_plan_f64 **plan, const char *recipe,
 #define MODEL_LANES 64
 #define MODEL_CACHE_LINE 128
 
+/* flops of one radix-r butterfly: the generated codelet's operation count
+ * (adds + multiplies of its straight-line code, emit.py codelet_flops) */
 static double codelet_flops(int radix) {
-  switch (radix) {
-    case 2: return 4; case 3: return 16; case 4: return 16; case 5: return 48;
-    case 6: return 57; case 7: return 96; case 8: return 56; case 9: return 121;
-    case 10: return 142; case 11: return 240; case 12: return 138;
-    case 13: return 336; case 14: return 259; case 15: return 276;
-    case 16: return 169; case 32: return 470; case 64: return 1192;
-    default: return -1;
-  }
+  if (radix < 2 || radix > 64 || !is_supported_radix(radix)) return -1;
+  const double f = mofft_codelet_flops[radix];
+  return f > 0 ? f : -1;
 }
 
 double mofft_estimated_radix_cost(int radix, int precision_bits,
@@ -2046,12 +2320,19 @@ double mofft_estimated_radix_cost(int radix, int precision_bits,
   return flops * MODEL_LANES / peak;
 }
 
+/* Memory tiers of the reference's cost profile (profiles/apple-m5-
+ * bootstrap.json tiers, src/mofft.c:133-168) re-measured on one MI355X with
+ * a 16-byte-per-lane copy / read / write probe (B/ns): HBM3E 5900 read,
+ * 5700 write, 5300 copy (1 GiB buffers); 128 MiB buffers (a copy's source
+ * and destination within the 256 MB Infinity Cache) 7600 copy.  LDS and L2
+ * are nominal (aggregate LDS 128 B/clk/CU x 256 CUs x 2.4 GHz; L2 twice
+ * the Infinity Cache). */
 static double model_memory_cost(size_t working_set, size_t read, size_t write) {
   static const struct { size_t cap; double read_bw, write_bw, mixed_bw, lat; } tier[] = {
-      {(size_t)160 << 10, 81920.0, 81920.0, 81920.0, 0.05},   /* LDS, all CUs */
-      {(size_t)32 << 20, 16000.0, 16000.0, 16000.0, 0.5},     /* 8 x 4 MiB L2 */
-      {(size_t)256 << 20, 10000.0, 10000.0, 10000.0, 1.0},    /* Infinity Cache */
-      {SIZE_MAX, 8000.0, 8000.0, 8000.0, 2.0},                /* HBM3E, B/ns */
+      {(size_t)160 << 10, 78643.0, 78643.0, 78643.0, 0.05},   /* LDS, all CUs */
+      {(size_t)32 << 20, 15200.0, 15200.0, 15200.0, 0.5},     /* 8 x 4 MiB L2 */
+      {(size_t)256 << 20, 7600.0, 7600.0, 7600.0, 1.0},       /* Infinity Cache */
+      {SIZE_MAX, 5900.0, 5700.0, 5300.0, 2.0},                /* HBM3E, B/ns */
   };
   for (size_t i = 0; i < sizeof(tier) / sizeof(tier[0]); ++i) {
     if (working_set <= tier[i].cap) {

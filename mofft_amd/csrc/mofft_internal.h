@@ -64,6 +64,19 @@ typedef struct {
   size_t twiddle_count;                /* complex elements */
 } mofft_schedule;
 
+/* Measured data of the analytical planner (generated mofft_costs.c):
+ * flops of one radix-r codelet (index r), and ns per byte moved of every
+ * pass-kernel registry entry as a single pass (column 0), a strided first
+ * pass at stride 16 / 64 / 256 / 1024 (columns 1-4) and a strided pass
+ * with pass twiddles at span 16 / 64 / 256 / 1024 (columns 5-8);
+ * 0 = not measured. */
+#define MOFFT_ROLE_ROWS 0
+#define MOFFT_ROLE_FIRST 1
+#define MOFFT_ROLE_LATER 2
+extern const double mofft_codelet_flops[65];
+extern const double mofft_pass_cost_ns[][9];
+extern const char *const mofft_pass_cost_source;
+
 /* Chain + pass mask -> passes over kernels of `ks` (greedy longest match
  * from the innermost stage; boundaries in `mask` always split). */
 mofft_status mofft_internal_build_schedule(const mofft_kernel_set *ks,

@@ -25,6 +25,13 @@
  *                   [--min-log2 A --max-log2 B] [--points P]
  *                   [--candidates C] [--samples S] [--no-sweep]
  *                   -o wisdom.json
+ *        mofft_tune --pass-costs passes.json [--precision ...] [--points P]
+ *
+ * --pass-costs measures the analytical planner's cost table instead: every
+ * compiled pass kernel timed in the roles it plays in plans, strided ones
+ * at four strides / spans (the planner's
+ * counterpart of the reference's calibrated cost profile,
+ * compiler/mofft_compiler/emitter.py:621-725, profiles/apple-m5-bootstrap.json).
  */
 #include <hip/hip_runtime.h>
 
@@ -233,6 +240,144 @@ void *plan_with_chain(int bits, size_t n, size_t batch, const Chain &c,
   return p;
 }
 
+/* Per-pass device seconds of one execution (median over `reps`), from
+ * the library's timed execute (events between pass launches). */
+bool pass_times(int bits, void *p, void *in, void *out, hipStream_t s, int reps,
+                std::vector<double> *secs) {
+  const int cap = 16;
+  std::vector<std::vector<double>> t(cap);
+  for (int i = 0; i < 3 + reps; ++i) {
+    double v[cap];
+    const mofft_status st =
+        bits == 32 ? mofft_execute_timed_f32((const mofft_plan_f32 *)p, (const mofft_complex_f32 *)in,
+                                             (mofft_complex_f32 *)out, s, v, cap)
+                   : mofft_execute_timed_f64((const mofft_plan_f64 *)p, (const mofft_complex_f64 *)in,
+                                             (mofft_complex_f64 *)out, s, v, cap);
+    if (st != MOFFT_SUCCESS) return false;
+    if (i >= 3)
+      for (int k = 0; k < cap; ++k) t[k].push_back(v[k]);
+  }
+  secs->clear();
+  for (int k = 0; k < cap; ++k) {
+    std::sort(t[k].begin(), t[k].end());
+    secs->push_back(t[k][t[k].size() / 2]);
+  }
+  return true;
+}
+
+/* The pass-cost table of the analytical planner: every registry kernel
+ * timed as the pass it is in plans, on `points` points (one HBM read and
+ * write of them per pass) - a whole-transform kernel as the single pass of
+ * N = L ("rows"); a strided kernel as the first pass of N = L x S with
+ * stride S ("first", no pass twiddles) and as the second pass of N = S x L
+ * with span S ("later", pass twiddles), for S = 16, 64, 256 and 1024 (the
+ * planner interpolates in log2 S), the other pass a bank-0 strided kernel
+ * of length S.  Each plan's description must name the kernel (else the
+ * case is skipped).  Writes JSON that mofft_compiler compiles into the
+ * library. */
+int pass_costs(const std::vector<int> &precisions, size_t points, int reps,
+               const char *path) {
+  struct Row {
+    int bits, rows, length, variant, span;
+    std::vector<int> stages;
+    const char *role;
+    double bytes, seconds;
+  };
+  std::vector<Row> rows;
+  hipStream_t stream = nullptr;
+  const int spans[] = {16, 64, 256, 1024};
+  for (int bits : precisions) {
+    const size_t cb = bits == 32 ? 8 : 16;
+    const size_t total = bits == 32 ? points : points / 2;
+    void *in = nullptr, *out = nullptr;
+    if (hipMalloc(&in, total * cb) != hipSuccess || hipMalloc(&out, total * cb) != hipSuccess) {
+      fprintf(stderr, "pass costs: allocation failed\n");
+      return 1;
+    }
+    if (mofft_server_generate(in, bits, total, 2026, stream) != MOFFT_SUCCESS ||
+        hipStreamSynchronize(stream) != hipSuccess)
+      return 1;
+    /* the strided pass of each span length the kernels are paired with */
+    std::vector<std::vector<int>> pair(4);
+    for (int q = 0; q < 4; ++q)
+      for (int i = 0; i < mofft_kernel_count(); ++i) {
+        mofft_kernel_info k;
+        if (mofft_kernel_get_info(i, &k) == MOFFT_SUCCESS && k.precision_bits == bits &&
+            !k.rows && k.length == spans[q] && k.variant == 0) {
+          pair[q].assign(k.stages, k.stages + k.stage_count);
+          break;
+        }
+      }
+    for (int i = 0; i < mofft_kernel_count(); ++i) {
+      mofft_kernel_info k;
+      if (mofft_kernel_get_info(i, &k) != MOFFT_SUCCESS || k.precision_bits != bits) continue;
+      std::vector<int> st(k.stages, k.stages + k.stage_count);
+      std::vector<int> inner(st.rbegin(), st.rend());     /* outer -> inner */
+      struct Case { const char *role; size_t n; Chain c; int slot; int span; };
+      std::vector<Case> cases;
+      if (k.rows) {
+        cases.push_back({"rows", (size_t)k.length, Chain{inner, 0}, 0, 1});
+      } else {
+        for (int q = 0; q < 4; ++q) {
+          if (pair[q].empty()) continue;
+          std::vector<int> pr(pair[q].rbegin(), pair[q].rend());
+          Chain first{pr, 0};
+          first.radices.insert(first.radices.end(), inner.begin(), inner.end());
+          first.mask = (uint64_t)1 << (pr.size() - 1);
+          Chain later{inner, 0};
+          later.radices.insert(later.radices.end(), pr.begin(), pr.end());
+          later.mask = (uint64_t)1 << (inner.size() - 1);
+          const size_t n = (size_t)k.length * (size_t)spans[q];
+          if (n > total) continue;
+          /* the wide bank widens only the passes after the first */
+          if (k.variant != 4) cases.push_back({"first", n, first, 0, spans[q]});
+          cases.push_back({"later", n, later, 1, spans[q]});
+        }
+      }
+      for (const Case &c : cases) {
+        const size_t batch = std::max<size_t>(1, total / c.n);
+        void *p = plan_with_chain(bits, c.n, batch, c.c, (unsigned)k.variant);
+        if (!p) continue;
+        mofft_plan_description d;
+        const mofft_status ds = bits == 32
+            ? mofft_plan_get_description_f32((const mofft_plan_f32 *)p, &d)
+            : mofft_plan_get_description_f64((const mofft_plan_f64 *)p, &d);
+        std::vector<double> secs;
+        if (ds == MOFFT_SUCCESS && d.pass_kernel[c.slot] == i &&
+            pass_times(bits, p, in, out, stream, reps, &secs)) {
+          rows.push_back({bits, k.rows, k.length, k.variant, c.span, st, c.role,
+                          2.0 * (double)(c.n * batch * cb), secs[c.slot]});
+        }
+        plan_destroy(bits, p);
+      }
+      if (i % 100 == 0) {
+        printf("pass costs: f%d kernel %d of %d\n", bits, i, mofft_kernel_count());
+        fflush(stdout);
+      }
+    }
+    hipFree(in);
+    hipFree(out);
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) return 1;
+  fprintf(f, "{\n \"schema_version\": 2,\n \"target\": \"gfx950\",\n");
+  fprintf(f, " \"source\": \"mofft_tune --pass-costs\",\n \"points\": %zu,\n", points);
+  fprintf(f, " \"build_id\": \"%s\",\n \"entries\": [\n", mofft_build_id());
+  for (size_t i = 0; i < rows.size(); ++i) {
+    const Row &r = rows[i];
+    fprintf(f, "  {\"precision\": %d, \"rows\": %d, \"length\": %d, \"stages\": [",
+            r.bits, r.rows, r.length);
+    for (size_t k = 0; k < r.stages.size(); ++k) fprintf(f, "%s%d", k ? ", " : "", r.stages[k]);
+    fprintf(f, "], \"variant\": %d, \"role\": \"%s\", \"span\": %d, \"bytes\": %.0f, "
+               "\"seconds\": %.9g}%s\n",
+            r.variant, r.role, r.span, r.bytes, r.seconds, i + 1 < rows.size() ? "," : "");
+  }
+  fprintf(f, " ]\n}\n");
+  fclose(f);
+  printf("wrote %zu pass costs to %s\n", rows.size(), path);
+  return 0;
+}
+
 std::vector<size_t> parse_list(const char *s) {
   std::vector<size_t> v;
   while (*s) {
@@ -255,6 +400,7 @@ int main(int argc, char **argv) {
   unsigned samples = 5;
   bool sweep = true;
   const char *out_path = "wisdom.json";
+  const char *pass_costs_path = nullptr;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&]() { return i + 1 < argc ? argv[++i] : (char *)""; };
@@ -268,6 +414,7 @@ int main(int argc, char **argv) {
     else if (a == "--candidates") candidates = strtoull(next(), nullptr, 10);
     else if (a == "--samples") samples = (unsigned)atoi(next());
     else if (a == "--no-sweep") sweep = false;
+    else if (a == "--pass-costs") pass_costs_path = next();
     else if (a == "-o") out_path = next();
     else { fprintf(stderr, "unknown argument %s\n", a.c_str()); return 2; }
   }
@@ -277,6 +424,7 @@ int main(int argc, char **argv) {
     fprintf(stderr, "mofft_tune: no gfx950 device\n");
     return 1;
   }
+  if (pass_costs_path) return pass_costs(precisions, points, (int)samples * 2, pass_costs_path);
   hipStream_t stream = nullptr;
   std::vector<Result> results;
   size_t mismatches = 0;
