@@ -17,9 +17,12 @@ Prints ONE JSON line:
              charged 1/P of that per pass launch.  The dominant kernel is
              the pass with the longest launches, timed with HIP events
              between consecutive pass launches on the launch stream over
-             the same number of executions as the timed region;
-             `achieved` = algorithmic bytes per launch / its mean launch
-             time.  `traffic` is the
+             the same number of executions as the timed region
+             (`event_launch_us`); its launch duration in the timed region
+             (`avg_launch_us`) is its share of the timed step in proportion
+             to those bracketed durations (an event pair adds a gap to the
+             launch it brackets); `achieved` = algorithmic bytes per launch
+             / avg_launch_us.  `traffic` is the
              measured HBM bytes per launch (rocprofv3 FETCH_SIZE+WRITE_SIZE,
              profiles/), or null when no counter summary is present.
   protocol   the reference benchmark's sampling (tools/benchmark.cpp:65-130)
@@ -80,6 +83,8 @@ def parse():
     ap.add_argument("--owned-tiles", type=int, default=0)
     ap.add_argument("--batch-bits", type=int, default=32, choices=[32, 64])
     ap.add_argument("--item-tiles", type=int, default=0)
+    ap.add_argument("--share-all", action="store_true",
+                    help="c4: every multi-pass transform shared between workgroups")
     ap.add_argument("--batch-grid", type=int, default=0,
                     help="c4: persistent grid size (0: resident workgroups per CU x CUs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -350,7 +355,13 @@ def run_gpu(args):
     kinfo = mf.kernels()[desc.pass_kernel[dom]]
     kernel_key = (f"f{bits}_L{kinfo.length}_c{kinfo.columns_per_block}_"
                   f"{'rows' if kinfo.rows else 'cols'}")
-    launch_us = pass_us[dom]
+    # The events between launches add their own gap to every launch they
+    # bracket (the same kernel runs 4-8 % shorter back to back, as in the
+    # timed region and in a kernel trace of it).  The dominant launch's
+    # duration in the timed region is its share of the timed step, in
+    # proportion to the bracketed durations of the step's launches.
+    step_launch_us = sum(pass_us) * chunks * len(plans)
+    launch_us = pass_us[dom] * (ms * 1e3) / step_launch_us
     achieved = alg_per_launch / (launch_us * 1e-6) / 1e9
     traffic = load_traffic(kernel_key)
     copy_gbs = measure_copy(torch, dev, stream, 256 << 20)
@@ -387,7 +398,7 @@ def run_gpu(args):
                                         + ", ".join(str(r) for r in kinfo.stages[:kinfo.stage_count]) + ">",
                      "launches_per_step": launches_per_step,
                      "avg_launch_us": round(launch_us, 3),
-                     "pass_launch_us": [round(v, 3) for v in pass_us]},
+                     "event_launch_us": [round(v, 3) for v in pass_us]},
         "wall_s": round(wall, 3),
         "protocol": proto,
     }
@@ -435,7 +446,8 @@ def run_batch(args):
            for n in lengths]
     outs = [a if ip else torch.empty_like(a) for a, ip in zip(ins, inplace)]
     bp = mf.BatchPlan(lengths, dirs, ins, outs, bits, graph=True, grid=args.batch_grid,
-                      owned_tiles=args.owned_tiles, item_tiles=args.item_tiles)
+                      owned_tiles=args.owned_tiles, item_tiles=args.item_tiles,
+                      share_all=args.share_all)
     d = bp.description()
 
     def timed(fn, reps):

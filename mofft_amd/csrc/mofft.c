@@ -647,8 +647,8 @@ static void search_splits(const mofft_kernel_set *ks, int precision_bits,
  * whole-transform kernel of any stage chain (tile banks 0, 1, 2, 4), else
  * the first bank-0 one compiled. */
 static int single_pass_kernel(const mofft_kernel_set *ks, int precision_bits,
-                              size_t length) {
-  if (ks != PASS_KERNELS || !pass_costs_measured())
+                              size_t length, int measured) {
+  if (ks != PASS_KERNELS || !measured)
     return find_kernel_by_length(ks, precision_bits, 1, length);
   int best = -1;
   double bn = 0.0;
@@ -716,12 +716,17 @@ static double chain_seconds(size_t length, int precision_bits, const int *radice
 
 /* Model plan: a single-pass kernel when one covers the length (with
  * measured costs the cheapest of its stage chains), otherwise the pass
- * decomposition of the lowest predicted HBM cost. */
-static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
-                                int precision_bits, int *radices,
-                                size_t *depth, uint64_t *mask) {
+ * decomposition of the lowest predicted HBM cost.  `use_measured` = 0
+ * gives the nominal plan: the first compiled (radix-16) chain of a single
+ * pass and the split of the nominal weights - the chains the matrix-core
+ * bank has kernels for. */
+static mofft_status model_chain_ex(const mofft_kernel_set *ks, size_t length,
+                                   int precision_bits, int *radices,
+                                   size_t *depth, uint64_t *mask,
+                                   int use_measured) {
   if (length < 2) return MOFFT_UNSUPPORTED_SIZE;
-  const int single = single_pass_kernel(ks, precision_bits, length);
+  const int measured = use_measured && pass_costs_measured();
+  const int single = single_pass_kernel(ks, precision_bits, length, measured);
   if (single >= 0) {
     const mofft_registry_entry *e = &ks->entries[single];
     for (int i = 0; i < e->stage_count; ++i)
@@ -731,21 +736,27 @@ static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
     return MOFFT_SUCCESS;
   }
   mofft_status st = best_split(ks, length, precision_bits, 0, radices, depth, mask);
-  if (st != MOFFT_SUCCESS || ks != PASS_KERNELS || !pass_costs_measured())
+  if (st != MOFFT_SUCCESS || ks != PASS_KERNELS || !measured)
     return st;
   int r2[MOFFT_MAX_PLAN_DEPTH];
   size_t d2 = 0;
   uint64_t m2 = 0;
   if (best_split(ks, length, precision_bits, 1, r2, &d2, &m2) != MOFFT_SUCCESS)
     return st;
-  const double nominal = chain_seconds(length, precision_bits, radices, *depth, *mask);
-  const double measured = chain_seconds(length, precision_bits, r2, d2, m2);
-  if (measured < MOFFT_MODEL_SPLIT_MARGIN * nominal) {
+  const double t_nominal = chain_seconds(length, precision_bits, radices, *depth, *mask);
+  const double t_measured = chain_seconds(length, precision_bits, r2, d2, m2);
+  if (t_measured < MOFFT_MODEL_SPLIT_MARGIN * t_nominal) {
     memcpy(radices, r2, d2 * sizeof(int));
     *depth = d2;
     *mask = m2;
   }
   return MOFFT_SUCCESS;
+}
+
+static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
+                                int precision_bits, int *radices,
+                                size_t *depth, uint64_t *mask) {
+  return model_chain_ex(ks, length, precision_bits, radices, depth, mask, 1);
 }
 
 /* The tile bank of the model plan: with measured costs, the plain bank
@@ -1614,7 +1625,10 @@ static mofft_status create_mfma(mofft_plan_common **out, size_t length,
   size_t depth = 0;
   uint64_t mask = 0;
   const double start = monotonic_seconds();
-  mofft_status st = model_chain(PASS_KERNELS, length, 32, radices, &depth, &mask);
+  /* the nominal chain: radix-16 sub-stages, whose kernels have
+   * matrix-core variants */
+  mofft_status st = model_chain_ex(PASS_KERNELS, length, 32, radices, &depth,
+                                   &mask, 0);
   if (st != MOFFT_SUCCESS) return st;
   st = plan_from_chain(out, length, batch, 32, direction, placement, radices,
                        depth, mask, MOFFT_MFMA_BANK);

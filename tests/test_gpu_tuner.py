@@ -55,12 +55,17 @@ def test_tuner_pass_cost_table(gpu, tmp_path):
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     t = json.loads(out.read_text())
-    assert t["schema_version"] == 1 and t["target"] == "gfx950"
+    assert t["schema_version"] == 2 and t["target"] == "gfx950"
     ks = [k for k in gpu.kernels() if k.precision_bits == 32]
     rows = [e for e in t["entries"] if e["role"] == "rows"]
     later = [e for e in t["entries"] if e["role"] == "later"]
     assert len(rows) == sum(1 for k in ks if k.rows)
-    assert len(later) == sum(1 for k in ks if not k.rows)
+    # every strided kernel as a later pass at every span its pairing fits in
+    # the 2^20 points of this run (L x span <= 2^20)
+    spans = (16, 64, 256, 1024)
+    assert len(later) == sum(1 for k in ks if not k.rows
+                             for s in spans if k.length * s <= 1 << 20)
     for e in t["entries"]:
         assert e["seconds"] > 0 and e["bytes"] > 0
         assert math.prod(e["stages"]) == e["length"]
+        assert e["role"] == "rows" or e["span"] in spans
