@@ -11,7 +11,10 @@ An entry is one instantiation of ``fft_pass_kernel`` (csrc/hip/fft_pass.h):
 * ``tpt`` threads per column and ``cols`` columns per workgroup;
 * ``variant``: the tile-shape family (the plan's kernel bank).  Bank 0 is
   the default shape; bank 1 doubles and bank 2 halves the columns per
-  workgroup, for power-of-two lengths >= 64 where the alternative fits.
+  workgroup, for power-of-two lengths >= 64 where the alternative fits;
+  bank 4 quadruples the columns of short strided tiles and bank 5 gives
+  long strided tiles more threads per column and fewer columns; bank 3 is
+  the matrix-core form.
 
 The reference's radix set {64, 32, 16, 15, ..., 2} (src/mofft.c:27-28) is
 covered: every radix has a strided single-radix entry (radix 32 and 64 run as
@@ -219,6 +222,7 @@ def _make(precision: int, length: int, rows: int,
 
 MFMA_VARIANT = 3
 WIDE_VARIANT = 4
+NARROW_VARIANT = 5
 
 
 def mfma_fits(e: Entry) -> bool:
@@ -234,10 +238,24 @@ def mfma_fits(e: Entry) -> bool:
 
 def _variants(e: Entry) -> list[Entry]:
     """Bank 1 (twice the columns) and bank 2 (half the columns) of a
-    power-of-two entry, where they fit the thread and LDS limits; bank 3:
+    power-of-two entry, where they fit the thread and LDS limits; bank 5
+    (narrow strided tiles, below); bank 3:
     the same tile with its radix-16 sub-stages on the matrix cores (FP32
     split-precision form), where the tile shape allows it."""
     out = []
+    if not e.rows and e.length >= 512 and e.length & (e.length - 1) == 0:
+        # bank 5: long strided columns with one butterfly of the largest
+        # radix per thread (at most 256 threads per column, at most 8
+        # points per thread) in 1024-thread workgroups - runs of 4-8
+        # adjacent columns instead of 8-16, so a thread holds half (a
+        # quarter) of the points and two workgroups can share a CU; the
+        # kernel deals the tiles of one cache line to one XCD
+        # (tile_of_block in csrc/hip/fft_pass.h)
+        tpt = min(e.length // max(e.stages), 256)
+        n = Entry(e.precision, e.length, 0, tpt, 1024 // tpt, e.stages, NARROW_VARIANT)
+        if (tpt > e.tpt and e.length // tpt <= 8 and n.cols >= 4
+                and n.lds_bytes <= LDS_CAP):
+            out.append(n)
     if not e.rows and 64 <= e.length <= 256 and e.length & (e.length - 1) == 0:
         # bank 4: four times the columns (512-byte+ runs per access)
         w = Entry(e.precision, e.length, e.rows, e.tpt, e.cols * 4, e.stages, WIDE_VARIANT)

@@ -23,7 +23,7 @@ from pathlib import Path
 RADICES = {2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 32, 64}
 
 
-PLAIN_BANKS = (0, 1, 2, 4)
+PLAIN_BANKS = (0, 1, 2, 4, 5)
 
 
 def load_entries(path: str | Path) -> tuple[list[dict], bool, str]:

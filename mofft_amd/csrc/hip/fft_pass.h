@@ -817,12 +817,39 @@ __device__ __forceinline__ void run_tile(const Ctx<T> &x,
   }
 }
 
+/* Tile of workgroup b among `blocks`.  A strided tile's row runs are
+ * C * sizeof(complex) bytes; below a 128-byte cache line, S = 128 / run
+ * adjacent tiles share every line they touch.  Workgroups are dealt to the
+ * eight XCDs in turn (b, b + 8, b + 16, ... run side by side on one XCD,
+ * whose L2 is its own), so with MOFFT_PASS_XCD_ORDER the S tiles of a line
+ * go to workgroups b, b + 8, ..., b + 8 (S - 1): each line is fetched into
+ * one L2 and written back whole from it.  A bijection on every full group
+ * of 8 S workgroups; the last partial group keeps its order. */
+template <typename T, int C, bool ROWS>
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t b, uint32_t blocks,
+                                                  uint32_t flags) {
+  constexpr int RUN = C * (int)sizeof(cx<T>);
+  if constexpr (ROWS || RUN >= 128 || 128 % RUN != 0) {
+    return b;
+  } else {
+    constexpr uint32_t S = 128 / RUN;
+    constexpr uint32_t G = 8 * S;
+    if (!(flags & MOFFT_PASS_XCD_ORDER)) return b;
+    const uint32_t g = b / G;
+    if ((g + 1) * G > blocks) return b;
+    const uint32_t r = b - g * G;
+    return g * G + (r % 8) * S + r / 8;
+  }
+}
+
 template <typename T, int L, int TPT, int C, bool ROWS, int R0, int... R>
 __global__ void __launch_bounds__(TPT * C)
 fft_pass_kernel(mofft_pass_args p) {
   extern __shared__ __align__(16) unsigned char mofft_lds[];
   Ctx<T> x;
-  setup_tile<T, L, TPT, C, ROWS>(x, p, (int64_t)blockIdx.x,
+  const uint32_t block =
+      tile_of_block<T, C, ROWS>(blockIdx.x, gridDim.x, p.flags);
+  setup_tile<T, L, TPT, C, ROWS>(x, p, (int64_t)block,
                                  reinterpret_cast<T *>(mofft_lds),
                                  (int)threadIdx.x);
   run_tile<T, L, TPT, C, ROWS, R0, R...>(x, p);

@@ -55,6 +55,7 @@
  * four-times-wider strided tiles. */
 #define MOFFT_MFMA_BANK 3u
 #define MOFFT_WIDE_BANK 4u
+#define MOFFT_NARROW_BANK 5u
 
 static size_t chunk_bytes(void) {
   static size_t cached = 0;
@@ -92,6 +93,19 @@ static size_t pass_table_bytes(int precision_bits) {
   }
   if (overridden) return v;
   return precision_bits == 64 ? MOFFT_PASS_TABLE_BYTES_F64 : MOFFT_PASS_TABLE_BYTES;
+}
+
+/* XCD-aware tile order of narrow strided tiles (MOFFT_PASS_XCD_ORDER);
+ * MOFFT_TILE_ORDER=linear in the environment turns it off (a tuning knob
+ * for comparing the two orders; read once per process).  The order never
+ * changes results. */
+static int xcd_tile_order(void) {
+  static int cached = -1;
+  if (cached < 0) {
+    const char *env = getenv("MOFFT_TILE_ORDER");
+    cached = !(env && strcmp(env, "linear") == 0);
+  }
+  return cached;
 }
 
 /* src/mofft.c:27-28 - the radix set the API accepts. */
@@ -644,7 +658,7 @@ static void search_splits(const mofft_kernel_set *ks, int precision_bits,
 }
 
 /* The single-pass kernel of a length: with measured costs the cheapest
- * whole-transform kernel of any stage chain (tile banks 0, 1, 2, 4), else
+ * whole-transform kernel of any stage chain (the plain tile banks), else
  * the first bank-0 one compiled. */
 static int single_pass_kernel(const mofft_kernel_set *ks, int precision_bits,
                               size_t length, int measured) {
@@ -760,12 +774,13 @@ static mofft_status model_chain(const mofft_kernel_set *ks, size_t length,
 }
 
 /* The tile bank of the model plan: with measured costs, the plain bank
- * (0, 1, 2, 4) whose schedule of the chain is predicted fastest; else 0. */
+ * (0, 1, 2, 4, 5) whose schedule of the chain is predicted fastest; else 0. */
 static unsigned int model_bank(size_t length, int precision_bits,
                                const int *radices, size_t depth,
                                uint64_t mask) {
   if (!pass_costs_measured()) return 0;
-  static const unsigned int banks[] = {0u, 1u, 2u, MOFFT_WIDE_BANK};
+  static const unsigned int banks[] = {0u, 1u, 2u, MOFFT_WIDE_BANK,
+                                       MOFFT_NARROW_BANK};
   unsigned int best = 0;
   double bt = 0.0;
   for (size_t i = 0; i < sizeof(banks) / sizeof(banks[0]); ++i) {
@@ -1003,6 +1018,7 @@ static void fill_pass_args(const mofft_plan_common *p, int i, const void *src,
   if (backward && i == P - 1) a->flags |= MOFFT_PASS_SWAP_OUT;
   if (ps->span > 1) a->flags |= MOFFT_PASS_TWIDDLE;
   if (ps->span > 1 && ps->pass_table) a->flags |= MOFFT_PASS_TWIDDLE_TABLE;
+  if (xcd_tile_order()) a->flags |= MOFFT_PASS_XCD_ORDER;
 }
 
 /* The passes of one chunk of `cnt` transforms starting at transform c0. */

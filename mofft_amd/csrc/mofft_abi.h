@@ -85,6 +85,9 @@ static inline void mofft_pass_args_set_divisors(mofft_pass_args *a,
 #define MOFFT_PASS_TWIDDLE 4u   /* apply the w_N pass twiddle on load */
 #define MOFFT_PASS_TWIDDLE_TABLE 8u  /* ... read from the coalesced full
                                         table instead of two-level lookups */
+#define MOFFT_PASS_XCD_ORDER 16u /* strided tiles narrower than a cache
+                                    line: workgroups on one XCD take the
+                                    tiles that share lines */
 
 /* One compiled pass kernel (host-side copy of the generated registry). */
 typedef struct {

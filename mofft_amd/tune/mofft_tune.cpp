@@ -469,7 +469,7 @@ int main(int argc, char **argv) {
       if (sweep) {
         double best = 0;
         for (const Chain &c : registry_chains(bits, n)) {
-          for (unsigned bank : {0u, 1u, 2u, 4u}) {
+          for (unsigned bank : {0u, 1u, 2u, 4u, 5u}) {
             void *p = plan_with_chain(bits, n, batch, c, bank);
             if (!p) continue;
             ++swept_count;
