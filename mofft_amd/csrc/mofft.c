@@ -99,13 +99,20 @@ static size_t pass_table_bytes(int precision_bits) {
  * MOFFT_TILE_ORDER=linear in the environment turns it off (a tuning knob
  * for comparing the two orders; read once per process).  The order never
  * changes results. */
+static int tile_order_override = -1;
+
 static int xcd_tile_order(void) {
   static int cached = -1;
+  if (tile_order_override >= 0) return tile_order_override;
   if (cached < 0) {
     const char *env = getenv("MOFFT_TILE_ORDER");
     cached = !(env && strcmp(env, "linear") == 0);
   }
   return cached;
+}
+
+void mofft_internal_set_tile_order(int xcd) {
+  tile_order_override = xcd < 0 ? -1 : xcd != 0;
 }
 
 /* src/mofft.c:27-28 - the radix set the API accepts. */

@@ -112,6 +112,10 @@ int mofft_internal_check_output(const void *out, const void *ref,
                                 double bound);
 void mofft_internal_describe(const mofft_schedule *s, size_t workspace_bytes,
                              mofft_plan_description *d);
+/* Tile order of the pass launches that follow (tests): 1 XCD-aware, 0
+ * linear, -1 the default (MOFFT_TILE_ORDER).  Results never depend on it;
+ * not for use while other threads launch. */
+void mofft_internal_set_tile_order(int xcd);
 
 /* Enqueue the first `count` (<= the plan's batch) transforms of a batched
  * plan (a mofft_plan_f32 * or mofft_plan_f64 *) on `stream`, device

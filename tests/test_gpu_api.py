@@ -135,6 +135,56 @@ def test_every_kernel_bank(gpu, n, bits):
         assert getattr(L, f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, count) == gpu.INVALID_ARGUMENT
 
 
+def _chain_plan(gpu, n, batch, bits, passes, bank):
+    """Plan whose passes run the given sub-stage chains (execution order,
+    first pass first) under kernel bank `bank`."""
+    chain, mask, acc = [], 0, 0
+    for pc in reversed(passes):
+        chain += list(reversed(pc))
+    for pc in reversed(passes[1:]):
+        acc += len(pc)
+        mask |= 1 << (acc - 1)
+    p = gpu.Plan(n, batch, bits, radices=chain, pass_mask=mask)
+    assert getattr(gpu.lib(), f"mofft_plan_set_kernel_bank_f{bits}")(p.handle, bank) == 0
+    return p
+
+
+@pytest.mark.parametrize("bits,n,batch,passes,bank", [
+    (32, 1 << 20, 2, [(8, 8, 4, 4), (8, 8, 4, 4)], 5),     # 64-byte runs
+    (32, 1 << 20, 2, [(16, 16, 4), (16, 16, 4)], 2),
+    (32, 1 << 22, 1, [(8, 8, 8, 4), (8, 8, 8, 4)], 5),     # 32-byte runs
+    (64, 1 << 20, 1, [(4, 4, 4, 4, 4), (4, 4, 4, 4, 4)], 5),
+    # tile counts that leave a partial group of 8 S workgroups
+    (32, 12 * 1024, 22, [(8, 8, 4, 4), (12,)], 5),
+    (32, 12 * 2048, 30, [(8, 8, 8, 4), (12,)], 5),
+])
+def test_tile_order_bit_identical(gpu, bits, n, batch, passes, bank):
+    """Strided tiles narrower than a 128-byte line are dealt to workgroups
+    so that the tiles sharing a line run on one XCD (MOFFT_PASS_XCD_ORDER).
+    The order moves whole tiles between workgroups, never the arithmetic:
+    the output is bit-identical to the linear order's and within the bar;
+    a tile dropped or run twice by the remap would show as a mismatch."""
+    L = gpu.lib()
+    dt = np.complex64 if bits == 32 else np.complex128
+    x = seeded_complex((batch, n), seed=n + bank).astype(dt)
+    outs = []
+    try:
+        with _chain_plan(gpu, n, batch, bits, passes, bank) as p:
+            d = p.description()
+            runs = [gpu.kernels()[d.pass_kernel[i]] for i in range(d.pass_count)]
+            assert any(not k.rows and k.columns_per_block * (bits // 4) < 128
+                       for k in runs), "no narrow tile in this plan"
+            for order in (1, 0):
+                L.mofft_internal_set_tile_order(order)
+                y = np.empty_like(x)
+                p.execute(x, y)
+                outs.append(y)
+    finally:
+        L.mofft_internal_set_tile_order(-1)
+    assert np.array_equal(outs[0], outs[1])
+    assert oracle.rel_l2(outs[0], oracle.fft_dd(x, -1)) <= tol(n, bits)
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 def test_recipe_roundtrip_replays_bits(gpu, bits):
     n = 20736
