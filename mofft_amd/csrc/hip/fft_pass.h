@@ -697,10 +697,15 @@ struct Stage {
       T ni[Next::IT][Next::RAD];
       /* a global table: the next sub-stage's twiddle loads are in flight
        * during the exchange instead of after it (FP32 N = 2^10..2^12 single
-       * pass 4-10 % faster).  Not in 1024-thread workgroups: a thread there
-       * has at most 128 VGPRs and the held twiddles spill (FP64 N = 2^14
-       * single pass 77 -> 127 us). */
-      constexpr bool PF = !TW_IN_LDS && Next::PREFETCH_TW && TPT * C < 1024;
+       * pass 4-10 % faster).  In 1024-thread workgroups a thread has at
+       * most 128 VGPRs: the held twiddles spill FP64 tiles (N = 2^14 single
+       * pass 77 -> 127 us) and cost strided FP32 tiles their second
+       * resident workgroup; FP32 whole-transform tiles up to 16384 points
+       * keep them (16384: 80 -> 104 VGPRs, one workgroup per CU either
+       * way). */
+      constexpr bool PF = !TW_IN_LDS && Next::PREFETCH_TW &&
+                          (TPT * C < 1024 ||
+                           (ROWS && sizeof(T) == 4 && L <= 16384));
       using NW = cx<T>[Next::IT][PF ? Next::RAD - 1 : 1];
       NW nw;
       if constexpr (PF) Next::load_tw(x, nw);
