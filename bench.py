@@ -564,13 +564,37 @@ def run_serve(args):
     return line
 
 
+def mfma32_chains(mf):
+    """Plan chains (outermost radix first) of the whole-transform
+    matrix-core kernels with radix-32 sub-stages, by length."""
+    out = {}
+    for k in mf.kernels():
+        st = list(k.stages[:k.stage_count])
+        if k.variant == mf.MFMA_BANK and k.rows and k.precision_bits == 32 and 32 in st:
+            out[k.length] = st[::-1]
+    return out
+
+
+def _sweep_plan(mf, form, n, batch, direction, chains):
+    if form != "mfma32":
+        return mf.Plan(n, batch, 32, direction, in_place=False, mfma=form == "mfma")
+    p = mf.Plan(n, batch, 32, direction, in_place=False, radices=chains[n])
+    if mf.lib().mofft_plan_set_kernel_bank_f32(p.handle, mf.MFMA_BANK) != 0:
+        raise RuntimeError(f"kernel bank {mf.MFMA_BANK} refused for N={n}")
+    p.set_blocked_transpose_mask(0)
+    return p
+
+
 def run_sweep(args, emit=True):
     """FP32 forward out-of-place, N = 2^k, batch = 2^24 / N: the plain FP32
-    form and the matrix-core split-precision form (one line per N and
-    form).  `rt_err` is the forward->backward round-trip relative L2 error
-    measured on the device (the oracle errors per N come from the tests)."""
+    form, the matrix-core split-precision form (radix-16 sub-stages) and,
+    for the lengths with such a kernel, the radix-32 matrix-core chain (one
+    line per N and form).  `rt_err` is the forward->backward round-trip
+    relative L2 error measured on the device (the oracle errors per N come
+    from the tests)."""
     import torch
     import mofft_amd as mf
+    chains = mfma32_chains(mf)
     dev = torch.device("cuda:0")
     sh = torch.cuda.current_stream(dev).cuda_stream
     g = torch.Generator(device=dev)
@@ -583,9 +607,9 @@ def run_sweep(args, emit=True):
     for k in range(4, args.sweep_max + 1):
         n = 1 << k
         batch = total // n
-        for form in ("plain", "mfma"):
-            p = mf.Plan(n, batch, 32, mf.FORWARD, in_place=False, mfma=form == "mfma")
-            q = mf.Plan(n, batch, 32, mf.BACKWARD, in_place=False, mfma=form == "mfma")
+        for form in ("plain", "mfma") + (("mfma32",) if n in chains else ()):
+            p = _sweep_plan(mf, form, n, batch, mf.FORWARD, chains)
+            q = _sweep_plan(mf, form, n, batch, mf.BACKWARD, chains)
             for _ in range(10):
                 p.execute_async(x, y, sh)
             torch.cuda.synchronize()
@@ -604,6 +628,7 @@ def run_sweep(args, emit=True):
             gbs = 2.0 * total * 8 / (ms * 1e-3) / 1e9
             row = {"workload": "fp32 forward out-of-place sweep", "form": form,
                    "length": n, "batch": batch, "passes": p.description().passes,
+                   "chain": p.description().chain,
                    "ms": round(ms, 4), "gflops": round(gf, 1),
                    "hbm_gbs": round(gbs, 1),
                    "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
@@ -652,6 +677,8 @@ def run_configs(args):
     sweep = {}
     for r in rows:
         e = sweep.setdefault(r["length"], {"passes": r["passes"]})
+        if r["form"] == "mfma32":
+            e["mfma32_chain"] = r["chain"]
         e[f"{r['form']}_ms"] = r["ms"]
         e[f"{r['form']}_hbm_frac"] = r["hbm_frac"]
         e[f"{r['form']}_rt_err"] = r["rt_err"]

@@ -29,7 +29,12 @@ extern "C" {
  * options->flags may add MOFFT_PLAN_MFMA (FP32, PRETUNED, no other flag):
  * the radix-16 sub-stages run on the matrix cores as split-precision FP16
  * products accumulated in FP32 (kernel bank 3; error stated per length by
- * the tests and the benchmark), with the analytical radix chain. */
+ * the tests and the benchmark), with the analytical radix chain.  Radix-32
+ * matrix-core sub-stages exist for the whole-transform chains 1024 =
+ * 32 x 32, 2048 = 4 x 16 x 32, 4096 = 4 x 32 x 32, 8192 = 16 x 16 x 32 and
+ * 16384 = 16 x 32 x 32 (radices outermost first): a plan made with that
+ * chain (mofft_plan_create_batched_with_radices_f32) on kernel bank 3
+ * (mofft_plan_set_kernel_bank_f32) runs them. */
 #define MOFFT_PLAN_MFMA 0x100u
 mofft_status mofft_plan_create_batched_f32(
     mofft_plan_f32 **plan, size_t length, size_t batch,

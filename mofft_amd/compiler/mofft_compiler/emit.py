@@ -65,40 +65,43 @@ def emit_codelet(r: int) -> tuple[str, dict]:
     return "\n".join(lines), op_count(outs)
 
 
-def mfma16_table() -> list[int]:
-    """FP16 bit patterns of the matrix-core radix-16 operand (csrc/hip/
-    fft_pass.h, MStage): the real 32x32 form M of the forward 16-point DFT
-    matrix (rows 0-15 real and 16-31 imaginary outputs, columns 0-15 real
-    and 16-31 imaginary inputs), split M = Mh + Ml into two FP16 values
-    (Mh = fp16(M), Ml = fp16(M - Mh), both rounded to nearest from the
-    exact cos/sin values), in A-operand lane order: entry
-    [part*2 + t][lane][e] = M{h,l}[lane % 32][16t + 8*(lane // 32) + e]
-    for part 0 (hi) / 1 (lo), t = 0 (real inputs) / 1 (imaginary)."""
+def mfma_table(radix: int) -> list[int]:
+    """FP16 bit patterns of the matrix-core radix-16 / radix-32 operand
+    (csrc/hip/fft_pass.h, MStage): the real 2R x 2R form M of the forward
+    R-point DFT matrix (rows 0..R-1 real and R..2R-1 imaginary outputs,
+    columns 0..R-1 real and R..2R-1 imaginary inputs), split M = Mh + Ml
+    into two FP16 values (Mh = fp16(M), Ml = fp16(M - Mh), both rounded to
+    nearest from the exact cos/sin values), in A-operand lane order of
+    R / 16 row tiles of 32 and R / 8 K-blocks of 16: entry
+    [part][rt][kb][lane][e] = M{h,l}[32 rt + lane % 32][16 kb + 8 (lane // 32) + e]
+    for part 0 (hi) / 1 (lo)."""
     import mpmath
     import numpy as np
     mpmath.mp.prec = 113
+    r2 = 2 * radix
 
-    def w(q, m):   # exp(-2 pi i q m / 16)
-        a = mpmath.mpf(2) * mpmath.pi * ((q * m) % 16) / 16
+    def w(q, m):   # exp(-2 pi i q m / R)
+        a = mpmath.mpf(2) * mpmath.pi * ((q * m) % radix) / radix
         return mpmath.cos(a), -mpmath.sin(a)
 
-    M = [[mpmath.mpf(0)] * 32 for _ in range(32)]
-    for q in range(16):
-        for m in range(16):
+    M = [[mpmath.mpf(0)] * r2 for _ in range(r2)]
+    for q in range(radix):
+        for m in range(radix):
             wr, wi = w(q, m)
             M[q][m] = wr
-            M[q][16 + m] = -wi
-            M[16 + q][m] = wi
-            M[16 + q][16 + m] = wr
+            M[q][radix + m] = -wi
+            M[radix + q][m] = wi
+            M[radix + q][radix + m] = wr
     out = []
     for part in (0, 1):
-        for t in (0, 1):
-            for lane in range(64):
-                for e in range(8):
-                    v = M[lane % 32][16 * t + 8 * (lane // 32) + e]
-                    hi = np.float16(float(v))
-                    val = hi if part == 0 else np.float16(float(v - mpmath.mpf(float(hi))))
-                    out.append(int(np.array(val, dtype=np.float16).view(np.uint16)))
+        for rt in range(radix // 16):
+            for kb in range(radix // 8):
+                for lane in range(64):
+                    for e in range(8):
+                        v = M[32 * rt + lane % 32][16 * kb + 8 * (lane // 32) + e]
+                        hi = np.float16(float(v))
+                        val = hi if part == 0 else np.float16(float(v - mpmath.mpf(float(hi))))
+                        out.append(int(np.array(val, dtype=np.float16).view(np.uint16)))
     return out
 
 
@@ -110,14 +113,15 @@ def emit_codelets_header() -> tuple[str, dict]:
              "  if constexpr (sizeof(T) == 4) return f; else return d;",
              "}", "",
              "template <int R> struct Dft;", ""]
-    tab = mfma16_table()
-    parts.append("/* matrix-core radix-16 operand: split FP16 DFT matrix, A-operand lane")
-    parts.append("   order [part * 2 + t][lane][e] (emit.py mfma16_table) */")
-    parts.append("static __constant__ __attribute__((aligned(16))) unsigned short "
-                 f"mofft_mfma16_a[{len(tab)}] = {{")
-    for i in range(0, len(tab), 16):
-        parts.append("  " + ", ".join(f"0x{v:04x}" for v in tab[i:i + 16]) + ",")
-    parts += ["};", ""]
+    for radix in (16, 32):
+        tab = mfma_table(radix)
+        parts.append(f"/* matrix-core radix-{radix} operand: split FP16 DFT matrix, A-operand")
+        parts.append("   lane order [part][row tile][K-block][lane][e] (emit.py mfma_table) */")
+        parts.append("static __constant__ __attribute__((aligned(16))) unsigned short "
+                     f"mofft_mfma{radix}_a[{len(tab)}] = {{")
+        for i in range(0, len(tab), 16):
+            parts.append("  " + ", ".join(f"0x{v:04x}" for v in tab[i:i + 16]) + ",")
+        parts += ["};", ""]
     counts = {}
     for r in codelets.RADICES:
         if r > 16:
@@ -131,8 +135,9 @@ def emit_codelets_header() -> tuple[str, dict]:
 
 def _targs(e: registry.Entry) -> str:
     t = "float" if e.precision == 32 else "double"
-    # matrix-core kernels (bank 3) mark their radix-16 sub-stages kMfma16
-    stages = ", ".join("kMfma16" if (e.variant == registry.MFMA_VARIANT and r == 16)
+    # matrix-core kernels (bank 3) mark their radix-16 / radix-32 sub-stages
+    # kMfma16 / kMfma32
+    stages = ", ".join(f"kMfma{r}" if (e.variant == registry.MFMA_VARIANT and r in (16, 32))
                        else str(r) for r in e.stages)
     return f"{t}, {e.length}, {e.tpt}, {e.cols}, {'true' if e.rows else 'false'}, {stages}"
 

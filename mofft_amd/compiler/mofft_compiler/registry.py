@@ -225,15 +225,55 @@ WIDE_VARIANT = 4
 NARROW_VARIANT = 5
 
 
+# Matrix-core chains with radix-32 sub-stages (bank 3 only: a VALU plan
+# runs a radix-32 level as a two-sub-stage L = 32 kernel), whole-transform
+# tiles: (length, chain, threads per column).  A radix-32 sub-stage is a
+# 64 x 64 real product per 32 butterflies: 2^10 .. 2^14 points need one LDS
+# exchange fewer than with radix-16 sub-stages.  Each chain has at least 32
+# butterflies per column in every sub-stage, so its tile loads and stores
+# HBM in the matrix layout.  (N = 2^15 as three radix-32 sub-stages holds
+# 64 values per lane in 1024-thread tiles and spills; not built.)
+MFMA32_ROWS = ((1024, (32, 32), 64), (2048, (32, 16, 4), 128),
+               (4096, (32, 32, 4), 256), (8192, (32, 16, 16), 512),
+               (16384, (32, 32, 16), 1024))
+
+
+def mfma_prefix(stages: tuple[int, ...]) -> int:
+    """Number of leading radix-16 / radix-32 sub-stages (the ones a bank-3
+    kernel runs on the matrix cores: they lead its chain)."""
+    n = 0
+    while n < len(stages) and stages[n] in (16, 32):
+        n += 1
+    return n
+
+
 def mfma_fits(e: Entry) -> bool:
-    """Tile shape constraints of the matrix-core radix-16 sub-stage: whole
-    waves, whole 32-butterfly groups, the same number of groups per wave."""
-    if e.precision != 32 or 16 not in e.stages:
+    """Tile shape constraints of the matrix-core sub-stages: FP32, radix-16
+    / radix-32 sub-stages only at the head of the chain, whole waves, whole
+    32-butterfly groups, the same number of groups per wave."""
+    k = mfma_prefix(e.stages)
+    if e.precision != 32 or k == 0:
         return False
-    nb = e.length // 16
-    waves, groups = e.threads // 64, (e.cols * nb) // 32
-    return (e.threads % 64 == 0 and (e.cols * nb) % 32 == 0 and waves > 0 and
-            groups % waves == 0)
+    if any(r in (16, 32) for r in e.stages[k:]):
+        return False
+    if e.threads % 64:
+        return False
+    waves = e.threads // 64
+    for r in e.stages[:k]:
+        nb = e.length // r
+        groups = (e.cols * nb) // 32
+        if (e.cols * nb) % 32 or groups % waves:
+            return False
+    return True
+
+
+def mfma32_entries() -> list[Entry]:
+    out = []
+    for length, chain, tpt in MFMA32_ROWS:
+        e = Entry(32, length, 1, tpt, max(1, 128 // tpt), chain, MFMA_VARIANT)
+        assert mfma_fits(e) and e.lds_bytes <= LDS_CAP, e
+        out.append(e)
+    return out
 
 
 def _variants(e: Entry) -> list[Entry]:
@@ -320,6 +360,7 @@ def entries() -> list[Entry]:
                 add(_make(precision, length, 0, chain))
         for k in range(1, BATCH_COLS_MAX.bit_length()):
             add(_make(precision, 1 << k, 0, BATCH_CHAIN(precision, 1 << k)))
+    out.extend(mfma32_entries())
     return out
 
 
@@ -388,8 +429,10 @@ def check(es: list[Entry]) -> None:
     for e in es:
         prod = 1
         for r in e.stages:
-            assert r in RADICES and r <= 16, e
+            assert r in RADICES and (r <= 16 or (r == 32 and e.variant == MFMA_VARIANT)), e
             prod *= r
+        if e.variant == MFMA_VARIANT:
+            assert mfma_fits(e), e
         assert prod == e.length, e
         assert e.threads <= 1024 and e.lds_bytes <= LDS_CAP, e
         # the cooperative tile store gives every thread the same number of

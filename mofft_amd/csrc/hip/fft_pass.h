@@ -312,10 +312,13 @@ __device__ __forceinline__ bool tile_store(const mofft_pass_args &p) {
          div_by((uint32_t)C, p.ns_magic, p.ns_shift) * ns == (uint32_t)C;
 }
 
-/* Radix value that marks a radix-16 sub-stage evaluated on the matrix
- * cores (MStage below) instead of by a VALU codelet. */
+/* Radix values that mark a radix-16 / radix-32 sub-stage evaluated on the
+ * matrix cores (MStage below) instead of by a VALU codelet.  In a kernel's
+ * chain the matrix-core sub-stages come first (the compiler's registry only
+ * builds such chains), so one scale per column covers all of them. */
 constexpr int kMfma16 = -16;
-constexpr int radix_value(int r) { return r == kMfma16 ? 16 : r; }
+constexpr int kMfma32 = -32;
+constexpr int radix_value(int r) { return r < 0 ? -r : r; }
 
 /* Sub-stage twiddle table of a kernel: entries OFF + (jj % NS) * (R - 1) + k
  * for every sub-stage with NS > 1, in execution order.  When it is small
@@ -351,7 +354,7 @@ template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
           int R, int... REST>
 struct Stage;
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
-          int... REST>
+          int RM, int... REST>
 struct MStage;
 
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
@@ -362,7 +365,12 @@ struct StageSel {
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
           int... REST>
 struct StageSel<T, L, TPT, C, ROWS, NS, OFF, kMfma16, REST...> {
-  using type = MStage<T, L, TPT, C, ROWS, NS, OFF, REST...>;
+  using type = MStage<T, L, TPT, C, ROWS, NS, OFF, 16, REST...>;
+};
+template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
+          int... REST>
+struct StageSel<T, L, TPT, C, ROWS, NS, OFF, kMfma32, REST...> {
+  using type = MStage<T, L, TPT, C, ROWS, NS, OFF, 32, REST...>;
 };
 
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
@@ -880,7 +888,7 @@ template <typename T, int L, int TPT, int C, bool ROWS, int... R>
 struct PassKernel {
   static constexpr int THREADS = TPT * C;
   static constexpr int STAGES = sizeof...(R);
-  static constexpr bool HAS_MFMA = ((R == kMfma16) || ...);
+  static constexpr bool HAS_MFMA = ((R < 0) || ...);
   static constexpr int LDS = StageTw<T, L, TPT, C, ROWS, R...>::BYTES;
   static int launch(const mofft_pass_args *a, hipStream_t s) {
     static std::atomic<unsigned long long> configured{0};
@@ -908,54 +916,145 @@ struct PassKernel {
 
 
 /* ------------------------------------------------------------------ */
-/* radix-16 sub-stage on the matrix cores (FP32 plans, kernel bank 3)   */
+/* radix-16 / radix-32 sub-stages on the matrix cores (FP32, bank 3)    */
 /* ------------------------------------------------------------------ */
 
+typedef _Float16 mf_h2 __attribute__((ext_vector_type(2)));
+typedef _Float16 mf_h4 __attribute__((ext_vector_type(4)));
 typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
+typedef float mf_f2 __attribute__((ext_vector_type(2)));
 typedef float mf_f16 __attribute__((ext_vector_type(16)));
 
-/* A radix-16 DFT of 32 butterflies is one real 32x32 by 32x32 product
- * D = M V: column j of V holds butterfly j's inputs (rows 0-15 real parts,
- * 16-31 imaginary parts), M is the real form of the 16-point DFT matrix
- * (mofft_mfma16_a, generated), column j of D the 16 outputs.  It runs as
- * v_mfma_f32_32x32x16_f16 products in split precision: every operand is
- * a sum hi + lo of two FP16 values, D = Mh Vh + Mh Vl + Ml Vh accumulated
- * in FP32 (the dropped Ml Vl is 2^-22 relative).  Each butterfly's inputs
- * are first scaled by a power of two that puts their largest magnitude in
- * [2^14, 2^15) (FP16 range for hi, normal range for lo); the outputs are
- * scaled back exactly.
+/* matrix-core sub-stages in a chain */
+template <int... RS>
+constexpr int mfma_stages() {
+  return (0 + ... + (RS < 0 ? 1 : 0));
+}
+
+/* log2 of the growth bound G of a chain's matrix-core sub-stages, taken at
+ * the first of them (RM, then REST): the inputs of the last one have
+ * components at most sqrt(2) times the product of the radices before it
+ * times the column's largest input component; G = 2 * that product. */
+template <int RM, int... REST>
+constexpr int mfma_growth_log2() {
+  constexpr int rs[] = {RM, REST..., 1};
+  const int n = 1 + mfma_stages<REST...>();
+  int lg = 1;
+  for (int i = 0; i + 1 < n; ++i)
+    for (int r = radix_value(rs[i]); r > 1; r >>= 1) ++lg;
+  return lg;
+}
+
+/* 2^j for j in [-126, 126] */
+__device__ __forceinline__ float exp2i(int j) {
+  return __uint_as_float((uint32_t)(j + 127) << 23);
+}
+
+/* Scale exponent k for values whose largest component magnitude is mx:
+ * mx * 2^k lies in [2^14, 2^15) / 2^lg (FP16 range for the high part,
+ * normal range for the low part), clamped so that 2^k and 2^-k are normal
+ * FP32 values (zero and subnormal mx give the upper clamp). */
+__device__ __forceinline__ int mfma_scale_exp(float mx, int lg) {
+  const int e = (int)((__float_as_uint(mx) >> 23) & 0xffu);
+  int k = 141 - lg - e;
+  k = k > 126 ? 126 : k;
+  k = k < -126 ? -126 : k;
+  return k;
+}
+
+/* a = ah + al, b = bh + bl in FP16, each part rounded to nearest (two
+ * values per conversion instruction) */
+__device__ __forceinline__ void split2(float a, float b, mf_h2 &hi,
+                                       mf_h2 &lo) {
+  const mf_f2 v = {a, b};
+  hi = __builtin_convertvector(v, mf_h2);
+  const mf_f2 h = __builtin_convertvector(hi, mf_f2);
+  lo = __builtin_convertvector(v - h, mf_h2);
+}
+
+__device__ __forceinline__ mf_h8 cat4(mf_h2 a, mf_h2 b, mf_h2 c, mf_h2 d) {
+  const mf_h4 ab = __builtin_shufflevector(a, b, 0, 1, 2, 3);
+  const mf_h4 cd = __builtin_shufflevector(c, d, 0, 1, 2, 3);
+  return __builtin_shufflevector(ab, cd, 0, 1, 2, 3, 4, 5, 6, 7);
+}
+
+/* LDS of a kernel's column scales: one exponent per column of the tile,
+ * and the partial maxima they are reduced from (one array per kernel,
+ * shared by its matrix-core sub-stages) */
+template <int C>
+__device__ __forceinline__ int *mfma_colexp() {
+  __shared__ int a[C];
+  return a;
+}
+template <int N>
+__device__ __forceinline__ float *mfma_slots() {
+  __shared__ float a[N];
+  return a;
+}
+
+/* A radix-R DFT (R = 16 or 32) of 32 butterflies is one real product
+ * D = M V: column j of V holds butterfly j's inputs (rows 0..R-1 real
+ * parts, R..2R-1 imaginary parts), M is the real 2R x 2R form of the
+ * R-point DFT matrix (mofft_mfma16_a / mofft_mfma32_a, generated), column j
+ * of D the R outputs.  It runs as v_mfma_f32_32x32x16_f16 products over
+ * R / 16 row tiles of 32 and R / 8 K-blocks of 16, in split precision:
+ * every operand is a sum hi + lo of two FP16 values and D = Ml Vh + Mh Vl
+ * + Mh Vh is accumulated in FP32 (the dropped Ml Vl is 2^-22 relative).
+ *
+ * Scaling: FP16 needs the inputs in a known range.  A chain with one
+ * matrix-core sub-stage scales each butterfly by the power of two that
+ * puts its largest input component in [2^14, 2^15) and scales the outputs
+ * back.  A chain with several (they lead the chain) scales each column
+ * once, at the first of them, by the power of two that keeps the inputs of
+ * all of them below 2^15 (the product of the radices before the last one
+ * bounds the growth), and scales back after the last: the low parts keep
+ * their precision down to 2^-3, some 2^18 below the largest value, so the
+ * error relative to the column stays that of the per-butterfly form.
  *
  * Operand layouts (gfx950, verified on the device): A[i][k] in lane
  * i%32 + 32*(k/8) element k%8; B[k][j] in lane j + 32*(k/8) element k%8;
  * D[i][j] in lane j + 32*((i/4)%2) register (i/8)*4 + i%4.  So lane
- * (j, h = lane/32) holds inputs m = 8h + e (e < 8) of butterfly j and,
- * after the products, outputs q = 8*(e/4) + 4h + e%4 (real in register e,
- * imaginary in register 8 + e).
+ * (j, h = lane/32) holds the R/2 inputs m = 16*(r/8) + 8h + r%8 and, after
+ * the products, the R/2 outputs q = 8*(r/4) + 4h + r%4 of butterfly j
+ * (r < R/2; real parts of the outputs in row tile 0 registers r (R = 32)
+ * or r (R = 16, r < 8), imaginary parts in row tile 1 or registers 8 + r).
  *
- * The stage reads its inputs from the LDS tile and leaves its outputs in
- * the tile like any other sub-stage; as the first sub-stage it loads from
- * HBM in the thread layout of a VALU radix-16 stage and exchanges through
- * LDS, as the last it exchanges back to that layout and stores from it. */
+ * Inputs come from the LDS tile and outputs go back to it like any other
+ * sub-stage's.  In a whole-transform (row) tile with at least 32
+ * butterflies per column, the first sub-stage loads its inputs from HBM in
+ * this layout and a last one stores from it (32 lanes: 32 consecutive
+ * elements).  Otherwise the first loads in the thread layout of a VALU
+ * radix-R stage and exchanges through LDS, the last exchanges back to that
+ * layout and stores from it. */
 template <typename T, int L, int TPT, int C, bool ROWS, int NS, int OFF,
-          int... REST>
+          int RM, int... REST>
 struct MStage {
   static_assert(sizeof(T) == 4, "matrix-core stages are FP32");
+  static_assert(RM == 16 || RM == 32, "matrix-core radix");
   using TL = Tile<T, L, TPT, C, ROWS>;
-  static constexpr int R = 16;
-  static constexpr int RAD = 8;            /* values per lane and group */
+  static constexpr int R = RM;
+  static constexpr int RAD = R / 2;        /* values per lane and group */
+  static constexpr int KB = R / 8;         /* K-blocks of 16 */
+  static constexpr int RT = R / 16;        /* row tiles of 32 */
   static constexpr int NB = L / R;
   static constexpr int NT = TPT * C;
   static constexpr int WAVES = NT / 64;
   static constexpr int GROUPS = (C * NB) / 32;
   static constexpr int IT = GROUPS / (WAVES > 0 ? WAVES : 1);
   static constexpr bool LAST = sizeof...(REST) == 0;
+  static constexpr bool FIRST_M = NS == 1;
+  static constexpr bool LAST_M = mfma_stages<REST...>() == 0;
+  static constexpr bool PER_BUTTERFLY = FIRST_M && LAST_M;
+  static constexpr bool DIRECT = ROWS && NB >= 32;
+  static constexpr int SPC = NB >= 32 ? NB / 32 : 1; /* slots per column */
   static constexpr bool PREFETCH_TW = false;
   static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
-      OFF + stage_tw_entries<NS, 16, REST...>());
+      OFF + stage_tw_entries<NS, RM, REST...>());
   static_assert(NT % 64 == 0 && (C * NB) % 32 == 0 && GROUPS % WAVES == 0,
                 "matrix-core tile shape");
-  /* the VALU radix-16 stage whose thread layout HBM loads / stores use */
-  using V = Stage<T, L, TPT, C, ROWS, NS, OFF, 16>;
+  /* the VALU radix-R stage whose thread layout HBM loads / stores use
+   * when they do not use this one's */
+  using V = Stage<T, L, TPT, C, ROWS, NS, OFF, R>;
 
   static __device__ __forceinline__ void where(const Ctx<T> &x, int it,
                                                int &c, int &jj, int &h) {
@@ -964,6 +1063,12 @@ struct MStage {
     c = g / NB;
     jj = g - c * NB;
     h = lane >> 5;
+  }
+  static __device__ __forceinline__ int in_m(int r, int h) {
+    return 16 * (r >> 3) + 8 * h + (r & 7);
+  }
+  static __device__ __forceinline__ int out_q(int r, int h) {
+    return 8 * (r >> 2) + 4 * h + (r & 3);
   }
 
   static __device__ __forceinline__ void load_lds(const Ctx<T> &x,
@@ -974,8 +1079,8 @@ struct MStage {
       int c, jj, h;
       where(x, it, c, jj, h);
 #pragma unroll
-      for (int e = 0; e < RAD; ++e)
-        v[it][e] = arr[TL::stage_addr(c, jj + (8 * h + e) * NB)];
+      for (int r = 0; r < RAD; ++r)
+        v[it][r] = arr[TL::stage_addr(c, jj + in_m(r, h) * NB)];
     }
   }
 
@@ -987,22 +1092,20 @@ struct MStage {
       where(x, it, c, jj, h);
       const int base = (jj / NS) * NS * R + (jj % NS);
 #pragma unroll
-      for (int e = 0; e < RAD; ++e) {
-        const int q = 8 * (e >> 2) + 4 * h + (e & 3);
-        arr[TL::stage_addr(c, base + q * NS)] = v[it][e];
-      }
+      for (int r = 0; r < RAD; ++r)
+        arr[TL::stage_addr(c, base + out_q(r, h) * NS)] = v[it][r];
     }
   }
 
   /* thread-layout values of V at their positions jj + k*NB */
   static __device__ __forceinline__ void put_natural(const Ctx<T> &x, T *arr,
-                                                     T (&u)[V::IT][16]) {
+                                                     T (&u)[V::IT][R]) {
 #pragma unroll
     for (int it = 0; it < V::IT; ++it) {
       const int jj = x.t + it * TPT;
       if (V::FULL || jj < NB) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k)
+        for (int k = 0; k < R; ++k)
           arr[TL::stage_addr(x.c, jj + k * NB)] = u[it][k];
       }
     }
@@ -1011,69 +1114,181 @@ struct MStage {
   static __device__ __forceinline__ void load_first(const Ctx<T> &x,
                                                     T (&vr)[IT][RAD],
                                                     T (&vi)[IT][RAD]) {
-    T ur[V::IT][16];
-    T ui[V::IT][16];
-    V::load_first(x, ur, ui);
-    put_natural(x, x.lds, ur);
+    if constexpr (DIRECT) {
+      /* every load issued before any is used */
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        int c, jj, h;
+        where(x, it, c, jj, h);
+        const uint32_t base = (uint32_t)c * L + (uint32_t)jj;
+        const bool ok = (uint32_t)c * L < x.tvalid;
+#pragma unroll
+        for (int r = 0; r < RAD; ++r) {
+          cx<T> v;
+          v.x = T(0);
+          v.y = T(0);
+          if (ok) v = ld_stream<T>(x.tsrc + base + (uint32_t)(in_m(r, h) * NB));
+          vr[it][r] = x.swap_in ? v.y : v.x;
+          vi[it][r] = x.swap_in ? v.x : v.y;
+        }
+      }
+    } else {
+      T ur[V::IT][R];
+      T ui[V::IT][R];
+      V::load_first(x, ur, ui);
+      put_natural(x, x.lds, ur);
+      __syncthreads();
+      load_lds(x, x.lds, vr);
+      __syncthreads();
+      put_natural(x, x.lds, ui);
+      __syncthreads();
+      load_lds(x, x.lds, vi);
+      __syncthreads();
+    }
+  }
+
+  /* Column scales at the first of several matrix-core sub-stages: every
+   * 32-lane group reduces the largest component magnitude of its columns
+   * into LDS slots, then every lane takes its column's maximum, records the
+   * exponent for the last sub-stage and scales its inputs. */
+  static __device__ __forceinline__ void column_scale(const Ctx<T> &x,
+                                                      T (&vr)[IT][RAD],
+                                                      T (&vi)[IT][RAD]) {
+    constexpr int W = NB >= 32 ? 32 : NB;  /* lanes of a column per half */
+    constexpr int LG = mfma_growth_log2<RM, REST...>();
+    float *slot = mfma_slots<C * SPC>();
+    int *cexp = mfma_colexp<C>();
+    const int lane = x.tid & 63;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int c, jj, h;
+      where(x, it, c, jj, h);
+      float mx = 0.f;
+#pragma unroll
+      for (int r = 0; r < RAD; ++r)
+        mx = fmaxf(mx, fmaxf(fabsf(vr[it][r]), fabsf(vi[it][r])));
+      mx = fmaxf(mx, __shfl_xor(mx, 32));
+#pragma unroll
+      for (int m = 1; m < W; m <<= 1) mx = fmaxf(mx, __shfl_xor(mx, m));
+      if (lane < 32 && jj % W == 0) slot[c * SPC + jj / 32] = mx;
+    }
     __syncthreads();
-    load_lds(x, x.lds, vr);
-    __syncthreads();
-    put_natural(x, x.lds, ui);
-    __syncthreads();
-    load_lds(x, x.lds, vi);
-    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      int c, jj, h;
+      where(x, it, c, jj, h);
+      float mx = 0.f;
+#pragma unroll
+      for (int k = 0; k < SPC; ++k) mx = fmaxf(mx, slot[c * SPC + k]);
+      const int e = mfma_scale_exp(mx, LG);
+      if (jj == 0 && h == 0) cexp[c] = e;
+      const float up = exp2i(e);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        vr[it][r] *= up;
+        vi[it][r] *= up;
+      }
+    }
   }
 
   static __device__ __forceinline__ void dft(const Ctx<T> &x, int it,
                                              T (&re)[RAD], T (&im)[RAD]) {
     int c, jj, h;
     where(x, it, c, jj, h);
-    if constexpr (NS > 1) {
-      const cx<T> *w = x.stw + OFF + (jj % NS) * (R - 1);
+    int k = 0;
+    if constexpr (PER_BUTTERFLY) {
+      float mx = 0.f;
 #pragma unroll
-      for (int e = 0; e < RAD; ++e) {
-        const int m = 8 * h + e;
-        if (m > 0) cmul<T>(re[e], im[e], w[m - 1]);
+      for (int r = 0; r < RAD; ++r)
+        mx = fmaxf(mx, fmaxf(fabsf(re[r]), fabsf(im[r])));
+      mx = fmaxf(mx, __shfl_xor(mx, 32));
+      k = mfma_scale_exp(mx, 0);
+      const float up = exp2i(k);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        re[r] *= up;
+        im[r] *= up;
       }
     }
-    float mx = 0.f;
-#pragma unroll
-    for (int e = 0; e < RAD; ++e)
-      mx = fmaxf(mx, fmaxf(fabsf(re[e]), fabsf(im[e])));
-    mx = fmaxf(mx, __shfl_xor(mx, 32));
-    int ex = 0;
-    (void)frexpf(mx, &ex);
-    int sh = 15 - ex;
-    sh = sh > 100 ? 100 : (sh < -100 ? -100 : sh);
-    const float up = ldexpf(1.f, sh);
-    const float down = ldexpf(1.f, -sh);
-    mf_h8 rh, rl, ih, il;
-#pragma unroll
-    for (int e = 0; e < RAD; ++e) {
-      const float a = re[e] * up;
-      const float b = im[e] * up;
-      const _Float16 ah = (_Float16)a;
-      const _Float16 bh = (_Float16)b;
-      rh[e] = ah;
-      ih[e] = bh;
-      rl[e] = (_Float16)(a - (float)ah);
-      il[e] = (_Float16)(b - (float)bh);
-    }
     const int lane = x.tid & 63;
-    const mf_h8 *tab = reinterpret_cast<const mf_h8 *>(mofft_mfma16_a);
-    const mf_h8 mh0 = tab[0 * 64 + lane], mh1 = tab[1 * 64 + lane];
-    const mf_h8 ml0 = tab[2 * 64 + lane], ml1 = tab[3 * 64 + lane];
-    mf_f16 acc = {};
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml0, rh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml1, ih, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh0, rl, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh1, il, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh0, rh, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh1, ih, acc, 0, 0, 0);
+    const mf_h8 *tab = reinterpret_cast<const mf_h8 *>(
+        R == 16 ? mofft_mfma16_a : mofft_mfma32_a);
+    mf_f16 acc[RT];
 #pragma unroll
-    for (int e = 0; e < RAD; ++e) {
-      re[e] = acc[e] * down;
-      im[e] = acc[8 + e] * down;
+    for (int rt = 0; rt < RT; ++rt) acc[rt] = mf_f16{};
+    /* twiddles and FP16 splits eight inputs at a time (K-blocks u of the
+     * real and u + KB/2 of the imaginary parts), then the products one row
+     * tile and one K-block at a time: the inputs die as they are split and
+     * only one block's matrix operands are live */
+    mf_h8 bh[KB], bl[KB];
+#pragma unroll
+    for (int u = 0; u < KB / 2; ++u) {
+      if constexpr (NS > 1) {
+        const cx<T> *w = x.stw + OFF + (jj % NS) * (R - 1);
+#pragma unroll
+        for (int r = 8 * u; r < 8 * u + 8; ++r) {
+          const int m = in_m(r, h);
+          if (m > 0) cmul<T>(re[r], im[r], w[m - 1]);
+        }
+      }
+#pragma unroll
+      for (int part = 0; part < 2; ++part) {
+        const int kb = u + part * (KB / 2);
+        const T *src = (part == 0 ? re : im) + 8 * u;
+        mf_h2 h0, h1, h2, h3, l0, l1, l2, l3;
+        split2(src[0], src[1], h0, l0);
+        split2(src[2], src[3], h1, l1);
+        split2(src[4], src[5], h2, l2);
+        split2(src[6], src[7], h3, l3);
+        bh[kb] = cat4(h0, h1, h2, h3);
+        bl[kb] = cat4(l0, l1, l2, l3);
+      }
+    }
+    /* the small terms (Ml Vh, Mh Vl, 2^-11 of the result) of every K-block
+     * first, the large ones last: each FP32 addition rounds at the scale of
+     * its partial sum, so only the KB additions of Mh Vh round at the
+     * result's scale */
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const mf_h8 ml = tab[((1 * RT + rt) * KB + kb) * 64 + lane];
+        acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml, bh[kb], acc[rt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const mf_h8 mh = tab[((0 * RT + rt) * KB + kb) * 64 + lane];
+        acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh, bl[kb], acc[rt], 0, 0, 0);
+      }
+#pragma unroll
+      for (int kb = 0; kb < KB; ++kb) {
+        const mf_h8 mh = tab[((0 * RT + rt) * KB + kb) * 64 + lane];
+        acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh, bh[kb], acc[rt], 0, 0, 0);
+      }
+      /* the next row tile's operand loads stay behind these products */
+      if constexpr (RT > 1) __builtin_amdgcn_sched_barrier(0);
+    }
+    if constexpr (RT == 1) {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        re[r] = acc[0][r];
+        im[r] = acc[0][8 + r];
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        re[r] = acc[0][r];
+        im[r] = acc[RT - 1][r];
+      }
+    }
+    if constexpr (PER_BUTTERFLY || (LAST_M && !FIRST_M)) {
+      if constexpr (!PER_BUTTERFLY) k = mfma_colexp<C>()[c];
+      const float down = exp2i(-k);
+#pragma unroll
+      for (int r = 0; r < RAD; ++r) {
+        re[r] *= down;
+        im[r] *= down;
+      }
     }
   }
 
@@ -1081,8 +1296,13 @@ struct MStage {
                                              const mofft_pass_args &p,
                                              T (&vr)[IT][RAD],
                                              T (&vi)[IT][RAD]) {
+    if constexpr (FIRST_M && !PER_BUTTERFLY) column_scale(x, vr, vi);
 #pragma unroll
-    for (int it = 0; it < IT; ++it) dft(x, it, vr[it], vi[it]);
+    for (int it = 0; it < IT; ++it) {
+      dft(x, it, vr[it], vi[it]);
+      /* one group's twiddles and operands live at a time */
+      if constexpr (IT > 1) __builtin_amdgcn_sched_barrier(0);
+    }
     if constexpr (!LAST) {
       using Next = typename StageSel<T, L, TPT, C, ROWS, NS * R,
                                      OFF + (NS > 1 ? NS * (R - 1) : 0),
@@ -1098,10 +1318,24 @@ struct MStage {
       Next::load_lds(x, x.lds, ni);
       __syncthreads();
       Next::run(x, p, nr, ni);
+    } else if constexpr (DIRECT) {
+      /* outputs q of butterfly jj at jj + q*NB (NS == NB here) */
+#pragma unroll
+      for (int it = 0; it < IT; ++it) {
+        int c, jj, h;
+        where(x, it, c, jj, h);
+        const uint32_t base = (uint32_t)c * L + (uint32_t)jj;
+        if ((uint32_t)c * L < x.tvalid) {
+#pragma unroll
+          for (int r = 0; r < RAD; ++r)
+            store_global<T>(x.tdst + base + (uint32_t)(out_q(r, h) * NB),
+                            vr[it][r], vi[it][r], x.swap_out);
+        }
+      }
     } else {
       /* back to V's thread layout (outputs at jj + q*NB), then store */
-      T ur[V::IT][16];
-      T ui[V::IT][16];
+      T ur[V::IT][R];
+      T ui[V::IT][R];
       put_lds(x, x.lds, vr);
       __syncthreads();
       V::load_lds(x, x.lds, ur);
@@ -1111,7 +1345,7 @@ struct MStage {
       V::load_lds(x, x.lds, ui);
       __syncthreads();
       if (ROWS || !tile_store<C>(p)) V::store_direct(x, ur, ui);
-      else store_tile<T, L, TPT, C, ROWS, V::IT, 16>(x, p, ur, ui);
+      else store_tile<T, L, TPT, C, ROWS, V::IT, R>(x, p, ur, ui);
     }
   }
 };

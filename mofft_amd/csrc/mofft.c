@@ -1649,7 +1649,9 @@ static mofft_status create_mfma(mofft_plan_common **out, size_t length,
   uint64_t mask = 0;
   const double start = monotonic_seconds();
   /* the nominal chain: radix-16 sub-stages, whose kernels have
-   * matrix-core variants */
+   * matrix-core variants (the radix-32 whole-transform kernels are slower
+   * at 1024-4096 and 16384 points and 6 % faster only at 8192; a plan
+   * selects them with their chain and kernel bank 3) */
   mofft_status st = model_chain_ex(PASS_KERNELS, length, 32, radices, &depth,
                                    &mask, 0);
   if (st != MOFFT_SUCCESS) return st;

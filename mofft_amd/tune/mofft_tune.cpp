@@ -233,6 +233,12 @@ void *plan_with_chain(int bits, size_t n, size_t batch, const Chain &c,
   if (st == MOFFT_SUCCESS)
     st = bits == 32 ? mofft_plan_set_kernel_bank_f32((mofft_plan_f32 *)p, bank)
                     : mofft_plan_set_kernel_bank_f64((mofft_plan_f64 *)p, bank);
+  /* regroup under the bank: a chain whose kernel exists only there (the
+   * radix-32 matrix-core whole-transform kernels) groups into fewer passes */
+  if (st == MOFFT_SUCCESS && bank != 0)
+    st = bits == 32
+        ? mofft_plan_set_blocked_transpose_mask_f32((mofft_plan_f32 *)p, c.mask)
+        : mofft_plan_set_blocked_transpose_mask_f64((mofft_plan_f64 *)p, c.mask);
   if (st != MOFFT_SUCCESS) {
     plan_destroy(bits, p);
     return nullptr;

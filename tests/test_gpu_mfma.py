@@ -1,6 +1,6 @@
 """GPU tests of the matrix-core (MFMA) split-precision form: FP32 plans whose
-radix-16 sub-stages run as FP16 hi/lo products accumulated in FP32
-(MOFFT_PLAN_MFMA, kernel bank 3).
+radix-16 (and, in the radix-32 chains, radix-32) sub-stages run as FP16
+hi/lo products accumulated in FP32 (MOFFT_PLAN_MFMA, kernel bank 3).
 
 The bar is the plain FP32 path's: relative L2 error against the FP64 oracle
 <= 1e-6*log2N for every N = 2^4..2^20 at batch 2^24/N (uniform[-1,1) inputs
@@ -130,3 +130,74 @@ def test_mfma_bank_on_reference_setter(gpu):
         q.execute(x, y2)
     assert np.array_equal(y, y2)
     assert oracle.rel_l2(y, oracle.dft(x.astype(np.complex128), -1)) <= tol(n)
+
+
+# radix-32 matrix-core chains (one whole-transform pass each), plan order
+# (outermost radix first)
+MFMA32 = [(1024, [32, 32]), (2048, [4, 16, 32]), (4096, [4, 32, 32]),
+          (8192, [16, 16, 32]), (16384, [16, 32, 32])]
+
+
+def _mfma_chain_plan(gpu, n, batch, radices, direction=None, in_place=False):
+    p = gpu.Plan(n, batch, 32, gpu.FORWARD if direction is None else direction,
+                 in_place=in_place, radices=radices)
+    assert gpu.lib().mofft_plan_set_kernel_bank_f32(p.handle, gpu.MFMA_BANK) == 0
+    # one pass: the chain's plain kernels group it into several (there is no
+    # plain whole-transform kernel with a radix-32 sub-stage)
+    p.set_blocked_transpose_mask(0)
+    return p
+
+
+@pytest.mark.parametrize("n,radices", MFMA32)
+def test_mfma_radix32_chains(gpu, n, radices):
+    """Radix-32 sub-stages on the matrix cores (64 x 64 real split-FP16
+    products): the plan is one pass of the radix-32 kernel; forward out of
+    place against the FP64 oracle, backward in place back to the input,
+    both within the plain FP32 bar; a unit impulse against its closed form."""
+    batch = max(2, (1 << 18) // n)
+    x = seeded_complex((batch, n), seed=n + 32, dtype=np.complex64)
+    ref = oracle.fft_dd(x.astype(np.complex128), -1)
+    with _mfma_chain_plan(gpu, n, batch, radices) as f, \
+            _mfma_chain_plan(gpu, n, batch, radices, gpu.BACKWARD, True) as b:
+        d = f.description()
+        assert d.passes == [n] and d.chain == radices
+        info = gpu.kernels()[d.pass_kernel[0]]
+        assert info.variant == gpu.MFMA_BANK
+        assert list(info.stages[:info.stage_count]) == radices[::-1]
+        y = np.empty_like(x)
+        f.execute(x, y)
+        fwd = oracle.rel_l2(y, ref)
+        z = y.copy()
+        b.execute(z, z)
+        rt = oracle.rel_l2(z / n, x)
+    n0 = int(np.random.default_rng(n).integers(0, n))
+    imp = np.zeros((1, n), np.complex64)
+    imp[0, n0] = 1
+    want = oracle.omega_table(n, -1, 64, indices=(np.arange(n, dtype=np.uint64) * n0) % n)
+    with _mfma_chain_plan(gpu, n, 1, radices) as fi:
+        yi = np.empty_like(imp)
+        fi.execute(imp, yi)
+    e_imp = oracle.rel_l2(yi[0], want)
+    print(f"N={n} radix-32 matrix-core {radices}: forward {fwd:.2e} round trip {rt:.2e} "
+          f"impulse {e_imp:.2e} bar {tol(n):.1e}")
+    assert fwd <= tol(n) and rt <= tol(n) and e_imp <= tol(n), (fwd, rt, e_imp)
+
+
+@pytest.mark.parametrize("n,radices", [(4096, [16, 16, 16]), (16384, [16, 32, 32]),
+                                       (1024, [32, 32]), (256, [16, 16])])
+def test_mfma_column_scale_per_transform(gpu, n, radices):
+    """Chains of several matrix-core sub-stages scale each column (each
+    transform of a whole-transform tile) once by its own power of two: a
+    batch whose transforms differ in magnitude by up to 2^100 either way
+    meets the bar transform by transform, not only over the batch."""
+    batch = 64
+    x = seeded_complex((batch, n), seed=n + 7, dtype=np.complex128)
+    ex = np.random.default_rng(n).integers(-100, 101, batch)
+    ex[0], ex[1] = -100, 100
+    x = (x * np.ldexp(1.0, ex)[:, None]).astype(np.complex64)
+    ref = oracle.fft_dd(x.astype(np.complex128), -1)
+    with _mfma_chain_plan(gpu, n, batch, radices) as f:
+        y = np.empty_like(x)
+        f.execute(x, y)
+    worst = max(oracle.rel_l2(y[i], ref[i]) for i in range(batch))
+    assert worst <= tol(n), worst

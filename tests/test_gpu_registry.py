@@ -52,6 +52,9 @@ def _run(gpu, n, batch, bits, direction, in_place, x, radices, mask, bank, kerne
     with gpu.Plan(n, batch, bits, direction, in_place, radices=radices, pass_mask=mask) as p:
         if bank:
             _set_bank(gpu, p, bits, bank)
+            # regroup under the bank: a chain whose whole-transform kernel
+            # exists only there (the radix-32 matrix-core ones) is one pass
+            p.set_blocked_transpose_mask(mask)
         d = p.description()
         assert d.pass_kernel[slot] == kernel, (n, radices, mask, bank, list(d.pass_kernel[:d.pass_count]))
         if in_place:
