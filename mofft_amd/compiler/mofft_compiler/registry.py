@@ -105,7 +105,8 @@ class Entry:
         most a quarter of that and the sum stays within LDS_CAP (the rule of
         StageTw in csrc/hip/fft_pass.h)."""
         real = self.complex_bytes // 2
-        lp = self.length + (self.length >> 4) + 1
+        # one pad element every 32 (FP32) / 16 (FP64): csrc/hip/fft_pass.h Tile
+        lp = self.length + (self.length >> (5 if self.precision == 32 else 4)) + 1
         row_words = self.cols * lp
         col_words = self.length * self.cols + ((self.length * self.cols) >> 5) + 1
         words = row_words if self.rows else max(row_words, col_words)

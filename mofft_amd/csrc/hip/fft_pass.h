@@ -68,12 +68,22 @@ template <> struct cx_of<float> { using type = float2; };
 template <> struct cx_of<double> { using type = double2; };
 template <typename T> using cx = typename cx_of<T>::type;
 
-/* LDS tile geometry (one array of T).  Column-major (ROWS) tiles pad one
- * word every 16; interleaved (COLS) tiles pad one word every 32. */
+/* LDS tile geometry (one array of T).  The LDS serves 128 bytes per cycle
+ * (32 banks of 4 bytes): G = 32 FP32 or 16 FP64 lanes of one access.
+ * ROWS tiles pad one element every G: a run of G consecutive elements then
+ * meets G different banks, and so do the stride-R writes of a first
+ * sub-stage (R a power of two up to 16).  (One element every 16 for FP32
+ * made even contiguous runs span 34 words, every access 2-way conflicted:
+ * FP32 8192 / 16384 single pass 7-8 % slower.  A per-exchange XOR swizzle
+ * also clears the middle sub-stages' writes, but its addresses are not
+ * affine in the element index, so the compiler holds one per element: up
+ * to 18 more VGPRs per thread, no faster.)  COLS tiles (interleaved
+ * columns) pad one word every 32. */
 template <typename T, int L, int TPT, int C, bool ROWS>
 struct Tile {
   static constexpr int NT = TPT * C;
-  static constexpr int LP = L + (L >> 4) + 1;
+  static constexpr int GSH = sizeof(T) == 4 ? 5 : 4;  /* log2 G */
+  static constexpr int LP = L + (L >> GSH) + 1;
   static constexpr int ROW_WORDS = C * LP;
   static constexpr int COL_WORDS = L * C + ((L * C) >> 5) + 1;
   static constexpr int WORDS = ROWS ? ROW_WORDS
@@ -83,7 +93,7 @@ struct Tile {
   static constexpr int BYTES = ARRAYS * WORDS * (int)sizeof(T);
 
   static __device__ __forceinline__ int row_addr(int c, int e) {
-    return c * LP + e + (e >> 4);
+    return c * LP + e + (e >> GSH);
   }
   static __device__ __forceinline__ int stage_addr(int c, int e) {
     if constexpr (ROWS) {
@@ -1045,7 +1055,9 @@ struct MStage {
   static constexpr bool FIRST_M = NS == 1;
   static constexpr bool LAST_M = mfma_stages<REST...>() == 0;
   static constexpr bool PER_BUTTERFLY = FIRST_M && LAST_M;
-  static constexpr bool DIRECT = ROWS && NB >= 32;
+  /* (not for tiles of 32+ points per thread: the 32768-point tile's
+   * direct loads spill 1000+ VGPRs, against 99 through the VALU layout) */
+  static constexpr bool DIRECT = ROWS && NB >= 32 && L / NT <= 16;
   static constexpr int SPC = NB >= 32 ? NB / 32 : 1; /* slots per column */
   static constexpr bool PREFETCH_TW = false;
   static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
