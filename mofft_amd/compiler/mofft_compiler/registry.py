@@ -42,6 +42,11 @@ MIXED_COLS = (121, 144, 169, 196, 225, 1728, 2197, 2744, 3375)
 # instead of one pass per radix stage
 SMALL_POWERS = (25, 27, 36, 49, 81, 100, 125, 216, 243, 343, 625, 729,
                 1000, 1296, 2187, 2401, 3125)
+# 3 x 2^k and 5 x 2^k lengths above 4096 (the autotuner's wisdom covers
+# them) as one HBM pass: FP32 up to 24576 points, FP64 up to 6144; radix-16
+# led and radix-8 led chains, the tuner picks
+LONG_SMOOTH_ROWS = {32: (5120, 6144, 10240, 12288, 20480, 24576),
+                    64: (5120, 6144)}
 SMOOTH_MAX = 1024          # single-pass and strided kernels
 SMOOTH_ROWS_MAX = 4096     # single-pass kernels only (a pass of a longer
                            # transform is at most 1024 points here)
@@ -335,6 +340,9 @@ def entries() -> list[Entry]:
             add(_make(precision, r, 1, (r,)))
         for length in MIXED_ROWS + SMALL_POWERS + smooth_lengths(SMOOTH_ROWS_MAX):
             add(_make(precision, length, 1, chain_for(length)))
+        for length in LONG_SMOOTH_ROWS[precision]:
+            add(_make(precision, length, 1, chain_for(length)))
+            add(_make(precision, length, 1, chain_for(length, cap=8)))
         for k in range(7, rows_max.bit_length()):
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
         for length, chain in ALT_RADIX4:

@@ -112,8 +112,9 @@ def test_no_device_means_unsupported_hardware():
     (161051, 64, [121, 121, 11]), (8748, 64, [108, 81]),
     # 1000-point FP32 strided kernels have 64-byte runs: three wide passes
     # (MI355X, 2^24 points: [125, 64, 125] 182-188 us, [125, 125, 64] 185,
-    # [64, 125, 125] 198; the nominal weights' [100, 100, 100] 199)
-    (10 ** 6, 32, [125, 64, 125]), (10 ** 6, 64, [1000, 1000]),
+    # [64, 125, 125] 198; the nominal weights' [100, 100, 100] 199): either
+    # of the two within the measurement's noise
+    (10 ** 6, 32, [[125, 64, 125], [125, 125, 64]]), (10 ** 6, 64, [1000, 1000]),
     # 2048-point passes cost three short ones
     (3 << 21, 32, [384, 128, 128]),
 ])
@@ -123,7 +124,11 @@ def test_model_schedule(n, bits, passes):
     split; the split the measured pass costs prefer where they predict it
     at least 15 % faster)."""
     d = mf.describe(n, 1, bits)
-    assert d.passes == passes
+    if isinstance(passes[0], list):
+        assert d.passes in passes
+        passes = d.passes
+    else:
+        assert d.passes == passes
     assert math.prod(d.chain) == n
     # the mask marks exactly the pass boundaries
     assert bin(d.pass_mask).count("1") == len(passes) - 1
