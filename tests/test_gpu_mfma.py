@@ -184,19 +184,27 @@ def test_mfma_radix32_chains(gpu, n, radices):
 
 
 @pytest.mark.parametrize("n,radices", [(4096, [16, 16, 16]), (16384, [16, 32, 32]),
-                                       (1024, [32, 32]), (256, [16, 16])])
+                                       (1024, [32, 32]), (256, [16, 16]),
+                                       (1 << 16, None), (1 << 18, None)])
 def test_mfma_column_scale_per_transform(gpu, n, radices):
     """Chains of several matrix-core sub-stages scale each column (each
-    transform of a whole-transform tile) once by its own power of two: a
-    batch whose transforms differ in magnitude by up to 2^100 either way
-    meets the bar transform by transform, not only over the batch."""
-    batch = 64
+    transform of a whole-transform tile, each strided column of a multi-pass
+    one) once by its own power of two: a batch whose transforms differ in
+    magnitude by up to 2^100 either way meets the bar transform by
+    transform, not only over the batch.  radices None: the MOFFT_PLAN_MFMA
+    plan (two strided passes of radix-16 pairs at 2^16 and 2^18)."""
+    batch = 64 if n <= 16384 else 8
     x = seeded_complex((batch, n), seed=n + 7, dtype=np.complex128)
     ex = np.random.default_rng(n).integers(-100, 101, batch)
     ex[0], ex[1] = -100, 100
     x = (x * np.ldexp(1.0, ex)[:, None]).astype(np.complex64)
     ref = oracle.fft_dd(x.astype(np.complex128), -1)
-    with _mfma_chain_plan(gpu, n, batch, radices) as f:
+    plan = (_mfma_chain_plan(gpu, n, batch, radices) if radices
+            else gpu.Plan(n, batch, 32, gpu.FORWARD, mfma=True))
+    with plan as f:
+        if radices is None:
+            assert len(f.description().passes) == 2
+            assert f.diagnostics.kernel_bank == gpu.MFMA_BANK
         y = np.empty_like(x)
         f.execute(x, y)
     worst = max(oracle.rel_l2(y[i], ref[i]) for i in range(batch))
