@@ -69,7 +69,8 @@ template <> struct cx_of<double> { using type = double2; };
 template <typename T> using cx = typename cx_of<T>::type;
 
 /* LDS tile geometry (one array of T).  The LDS serves 128 bytes per cycle
- * (32 banks of 4 bytes): G = 32 FP32 or 16 FP64 lanes of one access.
+ * (32 banks of 4 bytes; the gfx950 bank-conflict counters match this
+ * model): G = 32 FP32 or 16 FP64 lanes of one access.
  * ROWS tiles pad one element every G: a run of G consecutive elements then
  * meets G different banks, and so do the stride-R writes of a first
  * sub-stage (R a power of two up to 16).  (One element every 16 for FP32
@@ -1017,9 +1018,12 @@ __device__ __forceinline__ float *mfma_slots() {
  * back.  A chain with several (they lead the chain) scales each column
  * once, at the first of them, by the power of two that keeps the inputs of
  * all of them below 2^15 (the product of the radices before the last one
- * bounds the growth), and scales back after the last: the low parts keep
- * their precision down to 2^-3, some 2^18 below the largest value, so the
- * error relative to the column stays that of the per-butterfly form.
+ * bounds the growth), and scales back after the last.  The scaled column
+ * maximum is then at least 2^(14 - lg) (2^3 for the deepest chain,
+ * 32 x 32 x 16); values down to 2^-3 keep a normal low part, and below that
+ * the low part's absolute error (2^-25) is at most 2^-28 of the column
+ * maximum, so the error relative to the column stays that of the
+ * per-butterfly form (the power-of-two scalings themselves are exact).
  *
  * Operand layouts (gfx950, verified on the device): A[i][k] in lane
  * i%32 + 32*(k/8) element k%8; B[k][j] in lane j + 32*(k/8) element k%8;
