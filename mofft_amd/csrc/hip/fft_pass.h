@@ -1263,27 +1263,30 @@ struct MStage {
     /* the small terms (Ml Vh, Mh Vl, 2^-11 of the result) of every K-block
      * first, the large ones last: each FP32 addition rounds at the scale of
      * its partial sum, so only the KB additions of Mh Vh round at the
-     * result's scale */
+     * result's scale.  Consecutive products alternate between the row
+     * tiles' accumulators, so a radix-32 stage issues two independent
+     * chains instead of waiting on one product after another. */
 #pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
+    for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
+      for (int rt = 0; rt < RT; ++rt) {
         const mf_h8 ml = tab[((1 * RT + rt) * KB + kb) * 64 + lane];
         acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ml, bh[kb], acc[rt], 0, 0, 0);
       }
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
         const mf_h8 mh = tab[((0 * RT + rt) * KB + kb) * 64 + lane];
         acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh, bl[kb], acc[rt], 0, 0, 0);
       }
 #pragma unroll
-      for (int kb = 0; kb < KB; ++kb) {
+    for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
         const mf_h8 mh = tab[((0 * RT + rt) * KB + kb) * 64 + lane];
         acc[rt] = __builtin_amdgcn_mfma_f32_32x32x16_f16(mh, bh[kb], acc[rt], 0, 0, 0);
       }
-      /* the next row tile's operand loads stay behind these products */
-      if constexpr (RT > 1) __builtin_amdgcn_sched_barrier(0);
-    }
     if constexpr (RT == 1) {
 #pragma unroll
       for (int r = 0; r < RAD; ++r) {
@@ -1317,7 +1320,6 @@ struct MStage {
     for (int it = 0; it < IT; ++it) {
       dft(x, it, vr[it], vi[it]);
       /* one group's twiddles and operands live at a time */
-      if constexpr (IT > 1) __builtin_amdgcn_sched_barrier(0);
     }
     if constexpr (!LAST) {
       using Next = typename StageSel<T, L, TPT, C, ROWS, NS * R,
