@@ -296,11 +296,12 @@ static mofft_status validate_chain(size_t length, const int *radices,
 /* Group the chain into passes.  Level boundaries named in `mask` always
  * split; elsewhere each pass takes the longest run of stages that a compiled
  * kernel implements (greedy from the innermost stage). */
-static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
-                                   size_t batch, int precision_bits,
-                                   const int *radices, size_t depth,
-                                   uint64_t mask, unsigned int bank,
-                                   mofft_schedule *s) {
+static mofft_status build_schedule_layout(const mofft_kernel_set *ks,
+                                          size_t length, size_t batch,
+                                          int precision_bits,
+                                          const int *radices, size_t depth,
+                                          uint64_t mask, unsigned int bank,
+                                          int factored, mofft_schedule *s) {
   mofft_status st = validate_chain(length, radices, depth);
   if (st != MOFFT_SUCCESS) return st;
   if (batch == 0 || (precision_bits != 32 && precision_bits != 64))
@@ -316,6 +317,7 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   s->batch = batch;
   s->precision_bits = precision_bits;
   s->bank = bank;
+  s->factored_twiddles = factored;
   s->depth = (int)depth;
   memcpy(s->factors, radices, depth * sizeof(int));
   int exec[MOFFT_MAX_PLAN_DEPTH];
@@ -375,7 +377,7 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   for (int i = 1; i < s->pass_count; ++i) {
     mofft_pass *p = &s->passes[i];
     const size_t entries = p->span * p->length;
-    if (sat_mul(entries, cb0) <= pass_table_bytes(precision_bits)) {
+    if (!factored && sat_mul(entries, cb0) <= pass_table_bytes(precision_bits)) {
       p->pass_table = 1;
       p->pass_offset = count;
       count += entries;
@@ -406,6 +408,16 @@ static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
   if (s->chunk > max_chunk) s->chunk = max_chunk;
   s->workspace_bytes = s->pass_count > 1 ? sat_mul(s->chunk, transform_bytes) : 0;
   return MOFFT_SUCCESS;
+}
+
+/* The default twiddle layout: full pass tables up to pass_table_bytes. */
+static mofft_status build_schedule(const mofft_kernel_set *ks, size_t length,
+                                   size_t batch, int precision_bits,
+                                   const int *radices, size_t depth,
+                                   uint64_t mask, unsigned int bank,
+                                   mofft_schedule *s) {
+  return build_schedule_layout(ks, length, batch, precision_bits, radices,
+                               depth, mask, bank, 0, s);
 }
 
 /* Workspace is needed by in-place multi-pass plans and by out-of-place
@@ -952,27 +964,40 @@ static mofft_status make_tables(mofft_plan_common *p, const mofft_schedule *s) {
 }
 
 /* Build a plan from an explicit chain + mask (device resources included). */
-static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
-                                    size_t batch, int precision_bits,
-                                    int direction, mofft_placement placement,
-                                    const int *radices, size_t depth,
-                                    uint64_t mask, unsigned int bank) {
+static mofft_status plan_from_chain_layout(mofft_plan_common **out,
+                                           size_t length, size_t batch,
+                                           int precision_bits, int direction,
+                                           mofft_placement placement,
+                                           const int *radices, size_t depth,
+                                           uint64_t mask, unsigned int bank,
+                                           mofft_stage_layout layout) {
   *out = NULL;
   if (!mofft_hip_device_available()) return MOFFT_UNSUPPORTED_HARDWARE;
   mofft_plan_common *p = (mofft_plan_common *)calloc(1, sizeof(*p));
   if (!p) return MOFFT_ALLOCATION_FAILURE;
-  mofft_status st = build_schedule(PASS_KERNELS, length, batch, precision_bits,
-                                   radices, depth, mask, bank, &p->s);
+  mofft_status st = build_schedule_layout(
+      PASS_KERNELS, length, batch, precision_bits, radices, depth, mask, bank,
+      layout == MOFFT_STAGE_LAYOUT_SECTION, &p->s);
   if (st != MOFFT_SUCCESS) { free(p); return st; }
   p->direction = direction;
   p->placement = placement;
   p->direct_input = 0;
-  p->stage_layout = MOFFT_STAGE_LAYOUT_TRANSPOSE;
+  p->stage_layout = layout;
   p->kernel_bank = bank;
   st = make_tables(p, &p->s);
   if (st != MOFFT_SUCCESS) { plan_release(p); return st; }
   *out = p;
   return MOFFT_SUCCESS;
+}
+
+static mofft_status plan_from_chain(mofft_plan_common **out, size_t length,
+                                    size_t batch, int precision_bits,
+                                    int direction, mofft_placement placement,
+                                    const int *radices, size_t depth,
+                                    uint64_t mask, unsigned int bank) {
+  return plan_from_chain_layout(out, length, batch, precision_bits, direction,
+                                placement, radices, depth, mask, bank,
+                                MOFFT_STAGE_LAYOUT_TRANSPOSE);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1566,13 +1591,12 @@ static mofft_status create_with_options(mofft_plan_common **out,
     if (st != MOFFT_SUCCESS) return st;
     bank = model_bank(length, precision_bits, radices, depth, mask);
   }
-  st = plan_from_chain(out, length, batch, precision_bits, direction,
-                       placement, radices, depth, mask, bank);
+  st = plan_from_chain_layout(out, length, batch, precision_bits, direction,
+                              placement, radices, depth, mask, bank,
+                              w > 0 ? (mofft_stage_layout)layout
+                                    : MOFFT_STAGE_LAYOUT_TRANSPOSE);
   if (st != MOFFT_SUCCESS) return st;
-  if (w > 0) {
-    (*out)->direct_input = direct;
-    (*out)->stage_layout = (mofft_stage_layout)layout;
-  }
+  if (w > 0) (*out)->direct_input = direct;
   if (diagnostics) {
     diagnostics->used_wisdom = w > 0;
     diagnostics->used_model_fallback = w == 0;
@@ -2004,20 +2028,23 @@ mofft_status mofft_plan_get_description_f64(const mofft_plan_f64 *plan,
 /* ------------------------------------------------------------------ */
 
 /* Changing the pass grouping rebuilds the schedule and twiddle tables. */
-static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
-                            unsigned int bank) {
+static mofft_status regroup_layout(mofft_plan_common *p, uint64_t mask,
+                                   unsigned int bank,
+                                   mofft_stage_layout layout) {
   mofft_schedule s;
-  mofft_status st = build_schedule(PASS_KERNELS, p->s.length, p->s.batch,
-                                   p->s.precision_bits, p->s.factors,
-                                   (size_t)p->s.depth, mask, bank, &s);
+  mofft_status st = build_schedule_layout(
+      PASS_KERNELS, p->s.length, p->s.batch, p->s.precision_bits, p->s.factors,
+      (size_t)p->s.depth, mask, bank, layout == MOFFT_STAGE_LAYOUT_SECTION, &s);
   if (st != MOFFT_SUCCESS) return st;
-  int same = s.pass_count == p->s.pass_count;
+  int same = s.pass_count == p->s.pass_count &&
+             s.factored_twiddles == p->s.factored_twiddles;
   for (int i = 0; same && i < s.pass_count; ++i)
     same = s.passes[i].kernel == p->s.passes[i].kernel;
   if (same) {
     p->s.pass_mask = s.pass_mask;
     p->s.bank = bank;
     p->kernel_bank = bank;
+    p->stage_layout = layout;
     return MOFFT_SUCCESS;
   }
   /* graphs captured earlier keep their reference to the old tables */
@@ -2025,7 +2052,13 @@ static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
   if (st != MOFFT_SUCCESS) return st;
   p->s = s;
   p->kernel_bank = bank;
+  p->stage_layout = layout;
   return MOFFT_SUCCESS;
+}
+
+static mofft_status regroup(mofft_plan_common *p, uint64_t mask,
+                            unsigned int bank) {
+  return regroup_layout(p, mask, bank, p->stage_layout);
 }
 
 static mofft_status set_mask(mofft_plan_common *p, uint64_t mask) {
@@ -2067,8 +2100,13 @@ mofft_status mofft_plan_set_blocked_transpose_mask_f64(mofft_plan_f64 *plan,
 }
 
 /* direct_input and stage_layout select CPU data layouts in the reference
- * (src/mofft.c:1946-2009); on the GPU both layouts are the same LDS pass, so
- * the values are validated, kept, reported and exported only. */
+ * (src/mofft.c:1946-2009).  On the GPU direct_input is validated, kept,
+ * reported and exported only (both input layouts are the same LDS pass);
+ * stage_layout selects the layout of the pass twiddles: TRANSPOSE (the
+ * default) a full table per later pass up to a size limit (one coalesced
+ * load per point), SECTION the two-level factored table for every later
+ * pass (two small-table loads and a complex product per point; faster for
+ * some lengths, the autotuner records which). */
 mofft_status mofft_plan_set_direct_input_f32(mofft_plan_f32 *plan, int enabled) {
   if (!plan || (enabled != 0 && enabled != 1)) return MOFFT_INVALID_ARGUMENT;
   plan->c.direct_input = enabled;
@@ -2086,8 +2124,9 @@ mofft_status mofft_plan_set_stage_layout_f32(mofft_plan_f32 *plan,
   if (!plan || (layout != MOFFT_STAGE_LAYOUT_TRANSPOSE &&
                 layout != MOFFT_STAGE_LAYOUT_SECTION))
     return MOFFT_INVALID_ARGUMENT;
-  plan->c.stage_layout = layout;
-  return MOFFT_SUCCESS;
+  if (layout == plan->c.stage_layout) return MOFFT_SUCCESS;
+  return regroup_layout(&plan->c, plan->c.s.pass_mask, plan->c.kernel_bank,
+                        layout);
 }
 
 mofft_status mofft_plan_set_stage_layout_f64(mofft_plan_f64 *plan,
@@ -2095,8 +2134,9 @@ mofft_status mofft_plan_set_stage_layout_f64(mofft_plan_f64 *plan,
   if (!plan || (layout != MOFFT_STAGE_LAYOUT_TRANSPOSE &&
                 layout != MOFFT_STAGE_LAYOUT_SECTION))
     return MOFFT_INVALID_ARGUMENT;
-  plan->c.stage_layout = layout;
-  return MOFFT_SUCCESS;
+  if (layout == plan->c.stage_layout) return MOFFT_SUCCESS;
+  return regroup_layout(&plan->c, plan->c.s.pass_mask, plan->c.kernel_bank,
+                        layout);
 }
 
 /* A kernel bank is a tile-shape family (see kernel_bank_count); changing
@@ -2294,11 +2334,11 @@ static mofft_status import_plan(mofft_plan_common **out, const char *recipe,
   if (st != MOFFT_SUCCESS) return st;
   if (validate_chain(f.length, f.radices, f.depth) != MOFFT_SUCCESS)
     return MOFFT_INVALID_ARGUMENT;
-  st = plan_from_chain(out, f.length, 1, precision_bits, f.direction,
-                       f.placement, f.radices, f.depth, f.mask, f.bank);
+  st = plan_from_chain_layout(out, f.length, 1, precision_bits, f.direction,
+                              f.placement, f.radices, f.depth, f.mask, f.bank,
+                              f.layout);
   if (st != MOFFT_SUCCESS) { *out = NULL; return st; }
   (*out)->direct_input = f.direct_input;
-  (*out)->stage_layout = f.layout;
   if (diagnostics) {
     diagnostics->used_wisdom = 1;
     fill_diagnostics(diagnostics, *out);

@@ -62,6 +62,10 @@ typedef struct {
   size_t tw_hi_offset, tw_hi_count;
   size_t tw_lo_offset, tw_lo_count;
   size_t twiddle_count;                /* complex elements */
+  /* 1: every later pass reads its pass twiddles from the two-level
+   * (factored) table, whatever the full table's size (the plan's
+   * MOFFT_STAGE_LAYOUT_SECTION) */
+  int factored_twiddles;
 } mofft_schedule;
 
 /* Measured data of the analytical planner (generated mofft_costs.c):

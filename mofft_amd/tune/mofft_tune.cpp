@@ -18,8 +18,10 @@
  * lengths multiply to it (and, above 2^20 points, into three passes of at
  * least 64 points), each under every plain tile bank, timed the same way;
  * the fastest of the MEASURE winner, the sweep winner and the PRETUNED plan is
- * recorded, after its output for a seeded input has been checked against
- * the PRETUNED plan's (relative L2 within four times the precision's bound).
+ * recorded, in the faster of its two pass-twiddle layouts (stage_layout:
+ * full per-pass tables or the two-level table), after its output for a
+ * seeded input has been checked against the PRETUNED plan's (relative L2
+ * within four times the precision's bound).
  *
  * usage: mofft_tune [--precision 32|64|both] [--lengths N,N,...]
  *                   [--min-log2 A --max-log2 B] [--points P]
@@ -53,6 +55,7 @@ struct Result {
   std::vector<int> radices;
   uint64_t mask;
   unsigned bank;
+  int layout;  /* mofft_stage_layout: the pass-twiddle table layout */
   double seconds;
   double model_seconds;
   size_t batch;
@@ -74,6 +77,21 @@ void *plan_create(int bits, size_t n, size_t batch, const mofft_plan_options *o,
 void plan_destroy(int bits, void *p) {
   if (bits == 32) mofft_plan_destroy_f32((mofft_plan_f32 *)p);
   else mofft_plan_destroy_f64((mofft_plan_f64 *)p);
+}
+
+int pass_count(int bits, void *p) {
+  mofft_plan_description d;
+  const mofft_status st =
+      bits == 32 ? mofft_plan_get_description_f32((const mofft_plan_f32 *)p, &d)
+                 : mofft_plan_get_description_f64((const mofft_plan_f64 *)p, &d);
+  return st == MOFFT_SUCCESS ? (int)d.pass_count : 0;
+}
+
+bool set_layout(int bits, void *p, mofft_stage_layout layout) {
+  const mofft_status st =
+      bits == 32 ? mofft_plan_set_stage_layout_f32((mofft_plan_f32 *)p, layout)
+                 : mofft_plan_set_stage_layout_f64((mofft_plan_f64 *)p, layout);
+  return st == MOFFT_SUCCESS;
 }
 
 mofft_status run(int bits, void *p, void *in, void *out, hipStream_t s) {
@@ -530,6 +548,34 @@ int main(int argc, char **argv) {
         r.bank = win->kernel_bank;
       }
       r.seconds = tbest / 3;
+      /* the winner's pass-twiddle layout (mofft_plan_set_stage_layout:
+       * full per-pass tables or the two-level table) against the other,
+       * interleaved; the other is kept when it is at least 1% faster */
+      r.layout = chosen == model ? (int)dm.stage_layout
+                                 : chosen == measured ? (int)d.stage_layout
+                                                      : (int)MOFFT_STAGE_LAYOUT_TRANSPOSE;
+      if (pass_count(bits, chosen) > 1) {
+        const mofft_stage_layout cur = (mofft_stage_layout)r.layout;
+        const mofft_stage_layout alt = cur == MOFFT_STAGE_LAYOUT_SECTION
+                                           ? MOFFT_STAGE_LAYOUT_TRANSPOSE
+                                           : MOFFT_STAGE_LAYOUT_SECTION;
+        double tc = 0, ta = 0;
+        bool ok = true;
+        for (int round = 0; ok && round < 3; ++round) {
+          ok = set_layout(bits, chosen, cur);
+          if (ok) tc += steady(bits, chosen, in, out, stream, 15);
+          ok = ok && set_layout(bits, chosen, alt);
+          if (ok) ta += steady(bits, chosen, in, out, stream, 15);
+        }
+        if (ok && ta <= tc * 0.99) {
+          r.layout = (int)alt;
+          r.seconds = ta / 3;
+        }
+        if (!set_layout(bits, chosen, (mofft_stage_layout)r.layout)) {
+          fprintf(stderr, "N=%zu f%d: stage layout could not be set\n", n, bits);
+          return 1;
+        }
+      }
       if (model && chosen != model) {
         const double tol = (bits == 32 ? 1e-6 : 1e-15) * std::max(1.0, std::log2((double)n));
         const double dis = disagreement(bits, chosen, model, in, out, out2, n, batch, stream);
@@ -570,10 +616,11 @@ int main(int argc, char **argv) {
     for (size_t k = 0; k < r.radices.size(); ++k)
       fprintf(f, "%s%d", k ? ", " : "", r.radices[k]);
     fprintf(f, "], \"blocked_transpose_mask\": %llu, \"kernel_bank\": %u, "
-               "\"direct_input\": false, \"stage_layout\": \"transpose\", "
+               "\"direct_input\": false, \"stage_layout\": \"%s\", "
                "\"median_seconds\": %.9g, \"model_seconds\": %.9g, \"batch\": %zu}%s\n",
-            (unsigned long long)r.mask, r.bank, r.seconds, r.model_seconds, r.batch,
-            i + 1 < results.size() ? "," : "");
+            (unsigned long long)r.mask, r.bank,
+            r.layout == (int)MOFFT_STAGE_LAYOUT_SECTION ? "section" : "transpose",
+            r.seconds, r.model_seconds, r.batch, i + 1 < results.size() ? "," : "");
   }
   fprintf(f, " ]\n}\n");
   fclose(f);

@@ -206,6 +206,48 @@ def test_recipe_roundtrip_replays_bits(gpu, bits):
     assert np.array_equal(y0, y1)
 
 
+@pytest.mark.parametrize("bits,n", [(32, 1 << 20), (64, 1 << 16), (32, 1 << 18), (64, 3 << 14)])
+def test_stage_layout_selects_factored_pass_twiddles(gpu, bits, n):
+    """mofft_plan_set_stage_layout: SECTION makes every later pass read the
+    two-level (factored) pass-twiddle table instead of its full table, so
+    the plan's twiddle memory shrinks; the transform stays within the bar
+    against the oracle, and the layout survives export and import (the
+    imported plan replays the same bits)."""
+    batch = 2
+    dt = np.complex64 if bits == 32 else np.complex128
+    x = seeded_complex((batch, n), seed=n + bits).astype(dt)
+    L = gpu.lib()
+    setter = getattr(L, f"mofft_plan_set_stage_layout_f{bits}")
+    with gpu.Plan(n, batch, bits, gpu.FORWARD) as p:
+        d0 = p.description()
+        assert d0.pass_count >= 2
+        y0 = np.empty_like(x)
+        p.execute(x, y0)
+        assert setter(p.handle, 1) == gpu.SUCCESS          # MOFFT_STAGE_LAYOUT_SECTION
+        d1 = p.description()
+        assert list(d1.passes) == list(d0.passes) and d1.twiddle_bytes < d0.twiddle_bytes
+        y1 = np.empty_like(x)
+        p.execute(x, y1)
+        recipe = p.export()
+        assert setter(p.handle, 0) == gpu.SUCCESS          # back to TRANSPOSE
+        assert p.description().twiddle_bytes == d0.twiddle_bytes
+        y2 = np.empty_like(x)
+        p.execute(x, y2)
+    ref = oracle.fft_dd(x[:1].astype(np.complex128), -1)
+    assert oracle.rel_l2(y1[:1], ref) <= tol(n, bits)
+    assert oracle.rel_l2(y0[:1], ref) <= tol(n, bits)
+    assert np.array_equal(y0, y2)
+    h = ctypes.c_void_p()
+    d = gpu.PlanDiagnostics()
+    raw = recipe.encode()
+    st = getattr(L, f"mofft_plan_import_f{bits}")(ctypes.byref(h), raw, len(raw) + 1, ctypes.byref(d))
+    assert st == gpu.SUCCESS and d.stage_layout == 1
+    y3 = np.empty_like(x[:1])
+    assert getattr(L, f"mofft_execute_f{bits}")(h, x[:1].ctypes.data, y3.ctypes.data) == 0
+    getattr(L, f"mofft_plan_destroy_f{bits}")(h)
+    assert np.array_equal(y3, y1[:1])
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 @pytest.mark.parametrize("n,batch,in_place", [(30, 1, False), (4096, 16, True), (1 << 16, 4, False)])
 def test_measure_planner(gpu, n, batch, in_place, bits):
