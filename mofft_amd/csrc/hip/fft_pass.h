@@ -104,6 +104,40 @@ struct Tile {
       return q + (q >> 5);
     }
   }
+
+  /* stage_addr(c, e0 + K) == stage_addr(c, e0) + step(K) whenever
+   * (e0 mod 2^SH) + (K mod 2^SH) < 2^SH.  A row tile's pad term is
+   * e >> GSH; a column tile with C a power of two has q >> 5 =
+   * e >> (5 - log2 C) below 32 columns (c < C) and e * C / 32 + (c >> 5)
+   * from 32 columns on (SH = 0: linear in e).  The exchanges add the
+   * step as a compile-time offset instead of forming every element's
+   * padded address. */
+  static constexpr bool AFFINE = ROWS || (C & (C - 1)) == 0;
+  static constexpr int SH = ROWS ? GSH
+                                 : (C >= 32 ? 0
+                                            : 5 - (C >= 16 ? 4 : C >= 8 ? 3
+                                                   : C >= 4 ? 2 : C >= 2 ? 1 : 0));
+  static constexpr int step(int k) {
+    return ROWS ? k + (k >> GSH)
+                : (C >= 32 ? k * C + ((k * C) >> 5) : k * C + (k >> SH));
+  }
+  /* e0 = jj (any), K = k * NB */
+  static constexpr bool read_affine(int nb) {
+    return AFFINE && (SH == 0 || nb % (1 << SH) == 0);
+  }
+  /* e0 = (jj / ns) * ns * r + jj % ns, K = k * ns (k < r): exact when ns
+   * is a multiple of 2^SH, or a power of two dividing 2^SH with ns * r a
+   * multiple of 2^SH (then e0 mod 2^SH = jj % ns and K mod 2^SH <=
+   * 2^SH - ns) */
+  static constexpr bool write_affine(int ns, int r) {
+    return AFFINE &&
+           (SH == 0 || ns % (1 << SH) == 0 ||
+            ((ns & (ns - 1)) == 0 && (1 << SH) % ns == 0 &&
+             (ns * r) % (1 << SH) == 0));
+  }
+  /* row_addr(c, e0 + K) == row_addr(c, e0) + row_step(K) for K a
+   * multiple of 2^GSH */
+  static constexpr int row_step(int k) { return k + (k >> GSH); }
 };
 
 template <typename T>
@@ -258,9 +292,16 @@ __device__ __forceinline__ void store_tile(const Ctx<T> &x,
     for (int it = 0; it < IT; ++it) {
       const int jj = x.t + it * TPT;
       if (FULL || jj < NB) {
+        if constexpr (NB % (1 << TL::GSH) == 0) {
+          T *a = x.lds + TL::row_addr(x.c, jj);
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-          x.lds[TL::row_addr(x.c, jj + k * NB)] = half ? vi[it][k] : vr[it][k];
+          for (int k = 0; k < R; ++k)
+            a[TL::row_step(k * NB)] = half ? vi[it][k] : vr[it][k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            x.lds[TL::row_addr(x.c, jj + k * NB)] = half ? vi[it][k] : vr[it][k];
+        }
       }
     }
     __syncthreads();
@@ -613,10 +654,17 @@ struct Stage {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int jj = x.t + it * TPT;
+      if constexpr (TL::read_affine(NB)) {
+        const T *a = arr + TL::stage_addr(x.c, jj);
 #pragma unroll
-      for (int k = 0; k < R; ++k)
-        v[it][k] = (FULL || jj < NB) ? arr[TL::stage_addr(x.c, jj + k * NB)]
-                                     : T(0);
+        for (int k = 0; k < R; ++k)
+          v[it][k] = (FULL || jj < NB) ? a[TL::step(k * NB)] : T(0);
+      } else {
+#pragma unroll
+        for (int k = 0; k < R; ++k)
+          v[it][k] = (FULL || jj < NB) ? arr[TL::stage_addr(x.c, jj + k * NB)]
+                                       : T(0);
+      }
     }
   }
 
@@ -627,9 +675,15 @@ struct Stage {
       const int jj = x.t + it * TPT;
       if (FULL || jj < NB) {
         const int base = (jj / NS) * NS * R + (jj % NS);
+        if constexpr (TL::write_affine(NS, R)) {
+          T *a = arr + TL::stage_addr(x.c, base);
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-          arr[TL::stage_addr(x.c, base + k * NS)] = v[it][k];
+          for (int k = 0; k < R; ++k) a[TL::step(k * NS)] = v[it][k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            arr[TL::stage_addr(x.c, base + k * NS)] = v[it][k];
+        }
       }
     }
   }
@@ -1087,6 +1141,26 @@ struct MStage {
     return 8 * (r >> 2) + 4 * h + (r & 3);
   }
 
+  /* Exchange addresses as base + compile-time step (Tile::step): the
+   * lane half h goes into the base (jj + 8 h NB, resp. the butterfly's
+   * first output + 4 h NS), the rest of in_m / out_q into the step.  The
+   * output form is exact when no residue mod 2^SH carries, checked here
+   * over every j = jj mod NS, h and r. */
+  static constexpr bool put_affine() {
+    if (!TL::AFFINE) return false;
+    if (TL::SH == 0) return true;
+    const int m = 1 << TL::SH;
+    if ((NS * R) % m != 0 && NS % m != 0) return false;
+    for (int j = 0; j < NS; ++j)
+      for (int h = 0; h < 2; ++h)
+        for (int r = 0; r < RAD; ++r) {
+          const int e0 = (NS % m == 0) ? 0 : j + 4 * h * NS;
+          const int k = (8 * (r >> 2) + (r & 3)) * NS;
+          if (e0 % m + k % m >= m) return false;
+        }
+    return true;
+  }
+
   static __device__ __forceinline__ void load_lds(const Ctx<T> &x,
                                                   const T *arr,
                                                   T (&v)[IT][RAD]) {
@@ -1094,9 +1168,16 @@ struct MStage {
     for (int it = 0; it < IT; ++it) {
       int c, jj, h;
       where(x, it, c, jj, h);
+      if constexpr (TL::read_affine(NB)) {
+        const T *a = arr + TL::stage_addr(c, jj + 8 * h * NB);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r)
-        v[it][r] = arr[TL::stage_addr(c, jj + in_m(r, h) * NB)];
+        for (int r = 0; r < RAD; ++r)
+          v[it][r] = a[TL::step((16 * (r >> 3) + (r & 7)) * NB)];
+      } else {
+#pragma unroll
+        for (int r = 0; r < RAD; ++r)
+          v[it][r] = arr[TL::stage_addr(c, jj + in_m(r, h) * NB)];
+      }
     }
   }
 
@@ -1107,9 +1188,16 @@ struct MStage {
       int c, jj, h;
       where(x, it, c, jj, h);
       const int base = (jj / NS) * NS * R + (jj % NS);
+      if constexpr (put_affine()) {
+        T *a = arr + TL::stage_addr(c, base + 4 * h * NS);
 #pragma unroll
-      for (int r = 0; r < RAD; ++r)
-        arr[TL::stage_addr(c, base + out_q(r, h) * NS)] = v[it][r];
+        for (int r = 0; r < RAD; ++r)
+          a[TL::step((8 * (r >> 2) + (r & 3)) * NS)] = v[it][r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < RAD; ++r)
+          arr[TL::stage_addr(c, base + out_q(r, h) * NS)] = v[it][r];
+      }
     }
   }
 
@@ -1120,9 +1208,15 @@ struct MStage {
     for (int it = 0; it < V::IT; ++it) {
       const int jj = x.t + it * TPT;
       if (V::FULL || jj < NB) {
+        if constexpr (TL::read_affine(NB)) {
+          T *a = arr + TL::stage_addr(x.c, jj);
 #pragma unroll
-        for (int k = 0; k < R; ++k)
-          arr[TL::stage_addr(x.c, jj + k * NB)] = u[it][k];
+          for (int k = 0; k < R; ++k) a[TL::step(k * NB)] = u[it][k];
+        } else {
+#pragma unroll
+          for (int k = 0; k < R; ++k)
+            arr[TL::stage_addr(x.c, jj + k * NB)] = u[it][k];
+        }
       }
     }
   }
