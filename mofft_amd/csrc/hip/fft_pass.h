@@ -773,12 +773,13 @@ struct Stage {
        * pass 4-10 % faster).  In 1024-thread workgroups a thread has at
        * most 128 VGPRs: the held twiddles spill FP64 tiles (N = 2^14 single
        * pass 77 -> 127 us) and cost strided FP32 tiles their second
-       * resident workgroup; FP32 whole-transform tiles up to 16384 points
-       * keep them (16384: 80 -> 104 VGPRs, one workgroup per CU either
-       * way). */
+       * resident workgroup; FP32 whole-transform tiles up to 8192 points
+       * keep them.  The 16384-point tile without them needs 60 VGPRs (80
+       * with), so two of its 1024-thread workgroups share a CU: 63.2 ->
+       * 55.3 us per 2^24 points. */
       constexpr bool PF = !TW_IN_LDS && Next::PREFETCH_TW &&
                           (TPT * C < 1024 ||
-                           (ROWS && sizeof(T) == 4 && L <= 16384));
+                           (ROWS && sizeof(T) == 4 && L <= 8192));
       using NW = cx<T>[Next::IT][PF ? Next::RAD - 1 : 1];
       NW nw;
       if constexpr (PF) Next::load_tw(x, nw);
