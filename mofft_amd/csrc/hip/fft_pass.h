@@ -438,6 +438,19 @@ struct Stage {
   static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
       OFF + stage_tw_entries<NS, R, REST...>());
   static constexpr bool PREFETCH_TW = true;
+  /* FP32 row tiles of 16384 points and more keep their sub-stage table
+   * (about L entries) in global memory.  Their radix-16 / radix-8
+   * sub-stages load the twiddles of k = 1, 2, 3, 4, 8, 12 only and form
+   * each other one as the product w^(a j) w^(b j), k = a + b, a < 4, b a
+   * multiple of 4, of two correctly rounded table values (one rounding
+   * more).  The smaller set also fits the prefetch into the 64 VGPRs that
+   * two 1024-thread workgroups per CU allow: FP32 16384 single pass 55.3
+   * -> 49.9 us per 2^24 points. */
+  static constexpr bool SPARSE_TW =
+      sizeof(T) == 4 && ROWS && L >= 16384 && (R == 8 || R == 16);
+  static constexpr bool tw_loaded(int k) {
+    return !SPARSE_TW || k <= 4 || k % 4 == 0;
+  }
 
   /* this sub-stage's twiddles w_{NS R}^(j k), k = 1..R-1, of every
    * butterfly of the thread (j = jj % NS) */
@@ -454,7 +467,8 @@ struct Stage {
       if (FULL || jj < NB) {
         const cx<T> *src = x.stw + OFF + (jj % NS) * (R - 1);
 #pragma unroll
-        for (int k = 0; k < R - 1; ++k) w[it][k] = src[k];
+        for (int k = 0; k < R - 1; ++k)
+          if (tw_loaded(k + 1)) w[it][k] = src[k];
       }
     }
   }
@@ -756,8 +770,19 @@ struct Stage {
       if constexpr (NS > 1) {
         const int jj = x.t + it * TPT;
         if (FULL || jj < NB) {
+          if constexpr (SPARSE_TW) {
+            cx<T> ww[R - 1];
 #pragma unroll
-          for (int k = 1; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], w[it][k - 1]);
+            for (int k = 1; k < R; ++k)
+              ww[k - 1] = tw_loaded(k) ? w[it][k - 1]
+                                       : cprod<T>(w[it][k % 4 - 1],
+                                                  w[it][(k & ~3) - 1]);
+#pragma unroll
+            for (int k = 1; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], ww[k - 1]);
+          } else {
+#pragma unroll
+            for (int k = 1; k < R; ++k) cmul<T>(vr[it][k], vi[it][k], w[it][k - 1]);
+          }
         }
       }
       Dft<R>::template run<T>(vr[it], vi[it]);
@@ -774,12 +799,14 @@ struct Stage {
        * most 128 VGPRs: the held twiddles spill FP64 tiles (N = 2^14 single
        * pass 77 -> 127 us) and cost strided FP32 tiles their second
        * resident workgroup; FP32 whole-transform tiles up to 8192 points
-       * keep them.  The 16384-point tile without them needs 60 VGPRs (80
-       * with), so two of its 1024-thread workgroups share a CU: 63.2 ->
-       * 55.3 us per 2^24 points. */
+       * keep them, and the 16384-point tile where the next sub-stage loads
+       * the reduced set (SPARSE_TW): 60 VGPRs, so two of its 1024-thread
+       * workgroups share a CU (with the full set prefetched it needed 80,
+       * one workgroup per CU: 63.2 us per 2^24 points against 49.9). */
       constexpr bool PF = !TW_IN_LDS && Next::PREFETCH_TW &&
                           (TPT * C < 1024 ||
-                           (ROWS && sizeof(T) == 4 && L <= 8192));
+                           (ROWS && sizeof(T) == 4 &&
+                            (L <= 8192 || (L <= 16384 && Next::SPARSE_TW))));
       using NW = cx<T>[Next::IT][PF ? Next::RAD - 1 : 1];
       NW nw;
       if constexpr (PF) Next::load_tw(x, nw);
@@ -1119,6 +1146,7 @@ struct MStage {
   static constexpr bool DIRECT = ROWS && NB >= 32 && L / NT <= 16;
   static constexpr int SPC = NB >= 32 ? NB / 32 : 1; /* slots per column */
   static constexpr bool PREFETCH_TW = false;
+  static constexpr bool SPARSE_TW = false;
   static constexpr bool TW_IN_LDS = stage_tw_in_lds<T, L, TPT, C, ROWS>(
       OFF + stage_tw_entries<NS, RM, REST...>());
   static_assert(NT % 64 == 0 && (C * NB) % 32 == 0 && GROUPS % WAVES == 0,
