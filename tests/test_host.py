@@ -178,7 +178,7 @@ def test_pretuned_pass_counts(bits):
     transform is small, at least two once it cannot fit one workgroup's LDS
     (FP32 > 32768, FP64 > 16384 points), at most three up to 2^24; the
     passes multiply to N.  Which count wins in between is the tuner's
-    measurement (FP32 32768 is tuned to two passes, for instance)."""
+    measurement (FP32 32768 is tuned to one pass, 20480 to two, for instance)."""
     lds_max = 32768 if bits == 32 else 16384
     for k in range(3, 25):
         n = 1 << k
