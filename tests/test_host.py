@@ -108,7 +108,9 @@ def test_no_device_means_unsupported_hardware():
     (1 << 25, 32, [512, 256, 256]), (1 << 25, 64, [512, 256, 256]),
     (45, 32, [45]), (24, 64, [24]), (1000, 64, [1000]),
     (59049, 32, [243, 243]), (117649, 64, [343, 343]),
-    (100000, 32, [400, 250]), (33600, 32, [192, 175]),
+    (100000, 32, [400, 250]),
+    # MI355X, 2^24 points: [192, 175] 130-134 us, [105, 320] 134-135 us
+    (33600, 32, [[192, 175], [105, 320]]),
     (161051, 64, [121, 121, 11]), (8748, 64, [108, 81]),
     # 1000-point FP32 strided kernels have 64-byte runs: three wide passes
     # (MI355X, 2^24 points: [125, 64, 125] 182-188 us, [125, 125, 64] 185,
