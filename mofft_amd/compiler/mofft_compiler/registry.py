@@ -347,6 +347,10 @@ def entries() -> list[Entry]:
             add(_make(precision, 1 << k, 1, _pow2_chain8(1 << k)))
         for length, chain in ALT_RADIX4:
             add(_make(precision, length, 1, chain))
+        # 32 points as 4 x 4 x 2 with eight threads per transform and 32
+        # transforms per workgroup (FP32 2^24 points: 38.7 us against 41.0
+        # for the 8 x 4 tile, four threads per transform)
+        add(Entry(precision, 32, 1, 8, 32, (4, 4, 2)))
         # the persistent batch's chains, so a single-transform plan can
         # reproduce a batch transform exactly
         for k in range(1, BATCH_ROWS_MAX.bit_length()):
