@@ -351,6 +351,9 @@ def entries() -> list[Entry]:
         # transforms per workgroup (FP32 2^24 points: 38.7 us against 41.0
         # for the 8 x 4 tile, four threads per transform)
         add(Entry(precision, 32, 1, 8, 32, (4, 4, 2)))
+        # 128 points as 4 x 4 x 4 x 2, 32 threads per transform (38.8 us
+        # against 39.3 for the 8 x 4 x 4 tile)
+        add(Entry(precision, 128, 1, 32, 8, (4, 4, 4, 2)))
         # the persistent batch's chains, so a single-transform plan can
         # reproduce a batch transform exactly
         for k in range(1, BATCH_ROWS_MAX.bit_length()):
